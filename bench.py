@@ -1,0 +1,200 @@
+"""Benchmark: spin-steps/s of the DWI walker on one MI355X.
+
+Workload (one "step" of this benchmark = one full simulation pass):
+200 beaded axons (seeded synthetic substrate, ~7e4 spheres incl. images) in
+a 60 um periodic cube; 1e6 spins uniform in the box (intra/extra split by
+position), D = 2.0 um^2/ms everywhere; dt = 10 us, 5000 time steps; PGSE
+little_delta = 10 ms, big_delta = 40 ms at b = 1, 2, 3 ms/um^2 with 64
+Fibonacci-hemisphere directions per shell (192 signals).  Inputs are resident
+in HBM before timing; each timed pass re-initialises the spin state from the
+same start positions (nz_state_init), walks 5000 steps with phase in one
+launch, and reduces the 192 signals in FP64.
+
+Prints one JSON line (rank 0).  The CPU baseline is the FP64 oracle (OpenMP)
+on a bounded subsample of the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spins", type=int, default=1_000_000)
+    ap.add_argument("--nsteps", type=int, default=5000, help="time steps per simulation")
+    ap.add_argument("--axons", type=int, default=200)
+    ap.add_argument("--box", type=float, default=60.0)
+    ap.add_argument("--cell", type=float, default=0.0)
+    ap.add_argument("--margin", type=float, default=0.0)
+    ap.add_argument("--cpu-sample", type=int, default=20000,
+                    help="spins of the CPU-baseline subsample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=20261203)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
+        sys.exit(2)
+    import torch
+
+    from nozomi_amd import runtime as rt
+    from nozomi_amd.protocol import PGSEShells, fibonacci_hemisphere
+    from nozomi_amd.substrate import generate_beaded_axons, load_geometry
+
+    dev = torch.device("cuda:0")
+    M, L = generate_beaded_axons(args.axons, args.box, seed=args.seed)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    prot = PGSEShells(10.0, 40.0, 0.01, [1.0, 2.0, 3.0], fibonacci_hemisphere(64))
+    T = min(args.nsteps, prot.nsteps)
+    N = args.spins
+    sub = geom.device_substrate(cell_size=args.cell, margin=args.margin, device=dev)
+    info = sub.info()
+    st = rt.SpinState(N, dev, with_phase=True)
+    start = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=args.seed + 1)
+    g = torch.from_numpy(prot.g[:T].copy()).to(dev)
+    ga = torch.from_numpy(prot.garea[:T].copy()).to(dev)
+    dirs = torch.from_numpy(prot.dirs.astype(np.float32)).to(dev)
+    scales = torch.from_numpy(prot.scales.astype(np.float32)).to(dev)
+    sums = torch.empty((prot.nsignals, 2), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    walk_ms = []
+
+    def one_pass(record):
+        st.pos.copy_(start)
+        rt.state_init(sub, st)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        rt.walk(sub, st, args.seed, T, prot.dt, g=g, garea=ga, t2=False)
+        e1.record(stream)
+        rt.signal_sums(st.mom, dirs, scales, out=sums)
+        if record:
+            walk_ms.append((e0, e1))
+
+    for _ in range(args.warmup):
+        one_pass(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_pass(True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    walk_ms = [a.elapsed_time(b) for a, b in walk_ms]
+    signals = (sums[:, 0] / N).cpu().numpy()
+
+    spin_steps = float(N) * T * args.steps
+    value = spin_steps / elapsed
+    kern_ms = float(np.mean(walk_ms))
+    kern_rate = float(N) * T / (kern_ms * 1e-3)
+
+    # roofline of the walk kernel (dominant): FP32 vector work per spin-step
+    # (1 + s) * C * 56 + 40 with C = mean candidates per cell, s = mean
+    # scatter sub-steps per step (measured below on a subsample)
+    sbar = measure_sbar(geom, sub, prot, dev, args.seed)
+    cbar = info["ncand_mean_box"]
+    flop_per = (1.0 + sbar) * cbar * 56.0 + 40.0
+    achieved_tf = kern_rate * flop_per / 1e12
+    hbm_gbs = kern_rate * 24.0 / 1e9
+
+    out = {
+        "metric": "spin-steps/s at 1e6 spins",
+        "value": value,
+        "unit": "spin-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded beaded-axon substrate, uniform spins)",
+        "config": {
+            "workload": "plain walker FP32 + PGSE phase: 200 axons, 60 um box, 1e6 spins, "
+                        "5000 steps, 192 signals",
+            "spins": N, "time_steps": T, "dt_ms": prot.dt, "signals": prot.nsignals,
+            "spheres": int(info["nspheres"]), "structures": int(info["nstructures"]),
+            "cells": list(info["ncell"]), "cell_um": info["cell"][0], "margin_um": info["margin"],
+            "mean_candidates": cbar, "scatter_per_step": sbar,
+        },
+        "walk_kernel_ms": kern_ms,
+        "signals_per_s": prot.nsignals * args.steps / elapsed,
+        "roofline": {
+            "bound": "valu", "achieved": achieved_tf, "peak": FP32_VECTOR_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved_tf / FP32_VECTOR_PEAK_TFLOPS, "traffic": None,
+            "flop_per_spin_step": flop_per,
+        },
+        "roofline_hbm": {
+            "bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": hbm_gbs / HBM_PEAK_GBS, "bytes_per_spin_step": 24,
+        },
+        "signal_check": {"b1_mean": float(signals[:64].mean()), "b3_mean": float(signals[128:].mean())},
+    }
+    if args.cpu_sample > 0:
+        out["cpu_baseline"] = cpu_baseline(geom, prot, start, args)
+    print(json.dumps(out), flush=True)
+
+
+def measure_sbar(geom, sub, prot, dev, seed, n=65536, steps=200):
+    """Mean scatter sub-steps per spin-step on a subsample (GPU counter)."""
+    import torch
+
+    from nozomi_amd import runtime as rt
+    st = rt.SpinState(n, dev, with_phase=False, with_nsub=True)
+    sub.seed(n, targets=np.arange(geom.nstructures + 1), seed=seed + 7, out=st.pos)
+    rt.state_init(sub, st)
+    rt.walk(sub, st, seed, steps, prot.dt, t2=False)
+    return float(st.nsub.sum().item()) / (n * steps)
+
+
+def cpu_baseline(geom, prot, start, args):
+    """FP64 oracle (OpenMP over spins) on the first cpu_sample spins for
+    cpu_steps steps of the same workload."""
+    from oracle.oracle import OracleSubstrate, build
+    build()
+    n = min(args.cpu_sample, start.shape[1])
+    cores = len(os.sched_getaffinity(0))
+    threads = min(cores, 16)
+    orc = OracleSubstrate(geom.spheres, geom.nstructures, geom.structure_D(), geom.structure_T2(),
+                          (geom.Lx, geom.Ly, geom.Lz))
+    p = start[:, :n].cpu().numpy().astype(np.float64)
+    pos, pos0, sig, mom = p.copy(), p.copy(), np.ones(n), np.zeros((3, n))
+    T = min(args.cpu_steps, prot.nsteps)
+    t0 = time.perf_counter()
+    orc.walk(pos, pos0, sig, seed=args.seed, step0=0, nsteps=T, dt=prot.dt,
+             g=prot.g[:T].astype(np.float64), garea=prot.garea[:T].astype(np.float64), mom=mom,
+             nthreads=threads)
+    el = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": n * T / el, "unit": "spin-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} spins x {T} steps of the same workload (FP64 oracle, OpenMP)",
+            "cpu_model": model, "host_cores_available": cores}
+
+
+if __name__ == "__main__":
+    main()

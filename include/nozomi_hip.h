@@ -1,0 +1,159 @@
+/*
+ * nozomi_hip.h - C ABI of libnozomi_hip.so, the MI355X (gfx950) diffusion-MRI
+ * random walker.
+ *
+ * The reference drives its walker through PyCUDA: it %-formats the CUDA
+ * template simulation_toolkit/simulation_engine/sim3d_kernel.cu with the spin
+ * count, sphere count, box and per-structure D/T2 (diffsim3d.py:35-63,84-104),
+ * JIT-compiles it and launches one kernel per time step through
+ * get_function(...) handles.  This library replaces that "operator layer"
+ * with ordinary entry points: run-time parameters instead of baked-in
+ * constants, many steps per launch, and caller-owned device buffers.
+ *
+ * Conventions
+ *  - Every float / int pointer argument is a DEVICE pointer (for example
+ *    torch.Tensor.data_ptr() of a tensor on cuda:0) unless the argument
+ *    name ends in _host.
+ *  - Arrays are structure-of-arrays in the reference's layouts: [3, N]
+ *    positions are x[0..N), y[0..N), z[0..N) back to back.
+ *  - stream is a hipStream_t (NULL = the null stream).  Calls are asynchronous
+ *    on that stream except where stated; nothing synchronises implicitly.
+ *  - Return value: 0 on success, < 0 for an argument / shape / budget error,
+ *    > 0 for a hipError_t.  nz_last_error_string() gives the message
+ *    (thread-local).
+ *  - The library never frees caller memory.  It owns only the opaque
+ *    substrate handle (device copy of the spheres plus the cell grid).
+ */
+#ifndef NOZOMI_HIP_H
+#define NOZOMI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NZ_ABI_VERSION 1
+
+typedef struct nz_substrate* nz_substrate_t;
+
+/* Resident per-spin state.  All arrays are caller-owned device buffers.
+ * pos/pos0/sig mirror the reference's spins_d, spins0_d and sig_d
+ * (diffsim3d.py:43-44,62) and are rewritten by every walk call; the
+ * authoritative state between calls is the cell-relative part. */
+typedef struct nz_spin_state {
+  int64_t n;        /* number of spins                                      */
+  float* pos;       /* [3,n] absolute position, wrapped into the box (um)   */
+  float* pos0;      /* [3,n] start position shifted by every wrap (um)      */
+  float* sig;       /* [n]   T2 signal weight                               */
+  int32_t* ucell;   /* [3,n] unwrapped cell index per axis                  */
+  float* rel;       /* [3,n] position relative to the cell's lower corner   */
+  int32_t* ucell0;  /* [3,n] start cell (unwrapped, in [0, ncell_axis))     */
+  float* rel0;      /* [3,n] start position relative to that cell           */
+  float* logsig;    /* [n]   log of sig                                     */
+  int32_t* sid;     /* [n]   structure id of the spin after the last step   */
+  float* mom;       /* [3,n] phase moment sum_t g_t x_t (may be NULL)       */
+  int32_t* nsub;    /* [n]   scatter sub-steps so far (may be NULL)         */
+} nz_spin_state;
+
+/* Grid and substrate figures, filled by nz_substrate_info. */
+typedef struct nz_substrate_info_t {
+  int64_t nspheres;
+  int32_t nstructures;
+  int32_t ncell[3];        /* cells per axis inside the box              */
+  double cell[3];          /* cell edge per axis (um)                    */
+  double margin;           /* safety margin of the per-cell lists (um)   */
+  int64_t ncand_total;     /* candidate entries over all cells           */
+  int32_t ncand_max;       /* largest list                               */
+  double ncand_mean_box;   /* mean list length over the in-box cells     */
+  int64_t device_bytes;    /* device memory held by the handle           */
+} nz_substrate_info_t;
+
+int nz_abi_version(void);
+const char* nz_last_error_string(void);
+
+/* Upload a substrate and build its cell grid.
+ * Replaces SimGeometry3D.spheres upload + DiffSim3d.set_segments
+ * (diffsim3d.py:54,107-174, geometry.py:37-67).
+ *   spheres_host : [5, nspheres] row-major doubles (cx, cy, cz, r, sid) as
+ *                  SimGeometry3D.spheres; sid = structure index (precedence:
+ *                  lower wins, sim3d_kernel.cu:53-54)
+ *   D_host, T2_host : nstructures + 1 entries, the box last (diffsim3d.py:95-103)
+ *   L*       : box edges (um); periodic cube [-L/2, L/2]
+ *   cell_size: target cell edge (um, 0 = auto); the grid tiles the box exactly
+ *   margin   : per-cell list margin (um, 0 = auto from the jump length)
+ * Synchronous (builds on the host, copies to the current device). */
+int nz_substrate_create(const double* spheres_host, int64_t nspheres, int32_t nstructures,
+                        const float* D_host, const float* T2_host, double Lx, double Ly,
+                        double Lz, double cell_size, double margin, nz_substrate_t* out);
+int nz_substrate_destroy(nz_substrate_t sub);
+int nz_substrate_info(nz_substrate_t sub, nz_substrate_info_t* info);
+
+/* Rejection-seed n spins uniformly inside the target structures.
+ * Replaces SimGeometry3D.gpu_seed + seedSpinsKernel (geometry.py:73-174,
+ * sim3d_kernel.cu:240-337).  A point is accepted when it lies in a target
+ * structure and in no earlier non-target structure.  Attempt k of spin i
+ * draws Philox counter (k, 0, i, 1) under key seed.
+ *   targets_host : ntargets structure ids (box = nstructures)
+ *   pos          : [3, n] device output
+ *   nfailed      : device int32, receives the number of spins still unplaced
+ *                  after max_attempts (may be NULL) */
+int nz_seed(nz_substrate_t sub, int64_t n, const int32_t* targets_host, int32_t ntargets,
+            uint64_t seed, int32_t max_attempts, float* pos, int32_t* nfailed, void* stream);
+
+/* Build the resident state from absolute positions st->pos (device [3,n]):
+ * cell-relative coordinates, pos0 = pos, sig = 1, mom = 0, nsub = 0 and the
+ * structure id of every spin.  Replaces the state set-up of DiffSim3d.setup
+ * (diffsim3d.py:42-45,56-62); initstates (sim3d_kernel.cu:65-70) has no
+ * counterpart: the Philox stream is stateless. */
+int nz_state_init(nz_substrate_t sub, const nz_spin_state* st, void* stream);
+
+/* Advance every spin by nsteps time steps of length dt (ms).
+ * Replaces nsteps launches of randomWalk3d / randomWalk3d_phase /
+ * randomWalk3d_phase_multidirr (sim3d_kernel.cu:340-943, launched by
+ * diffsim3d.py:199-226) with one persistent launch.
+ *   seed   : Philox key; draw k of step s of spin i uses counter (k, s, i, 0)
+ *   step0  : global index of the first step (continues the stream)
+ *   g, garea : device [nsteps] per-step phase coefficient and wrap area,
+ *              i.e. the G and G_area arguments of the reference launches
+ *              (diffsim3d.py:253-255); NULL for a walk without phase.
+ *              With them, st->mom accumulates sum_t g_t x_t with the
+ *              reference's wrap correction (sim3d_kernel.cu:674-715,880-942).
+ *   stepmom: device double [nsteps, 6] or NULL; receives the sums over spins
+ *            of dx^2, dy^2, dz^2, dx^4, dy^4, dz^4 after every step (the
+ *            moment sums of computeDiffusionCoefficientsAndKurtosis,
+ *            sim3d_kernel.cu:162-238), accumulated with atomics.
+ *   flags  : NZ_WALK_* bits. */
+#define NZ_WALK_T2 1u          /* update logsig / sig (randomWalk3d's T2 weight) */
+int nz_walk(nz_substrate_t sub, const nz_spin_state* st, uint64_t seed, int64_t step0,
+            int32_t nsteps, float dt, const float* g, const float* garea, double* stepmom,
+            uint32_t flags, void* stream);
+
+/* Sums over spins of dx^2, dy^2, dz^2, dx^4, dy^4, dz^4 of the current
+ * displacement (position minus spins0, i.e. the unwrapped displacement since
+ * nz_state_init): the block sums of computeDiffusionCoefficientsAndKurtosis
+ * (sim3d_kernel.cu:162-238) before the host scaling D = S/(2tN), K = S/N.
+ * out: device double [6], overwritten. */
+int nz_displacement_moments(nz_substrate_t sub, const nz_spin_state* st, double* out, void* stream);
+
+/* Phases of W directions from the moments: phase[w, i] = scale[w] *
+ * (dirs[0,w] m_x + dirs[1,w] m_y + dirs[2,w] m_z).  Replaces the per-step
+ * W-wide accumulation of randomWalk3d_phase_multidirr (sim3d_kernel.cu:880-942);
+ * the result equals DwiSim3d.simulate_multi_directions (diffsim3d.py:246-261).
+ *   mom [3,n], dirs [3,W], scale [W] (NULL = 1), phase out [W, n]. */
+int nz_project_phase(int64_t n, int32_t W, const float* mom, const float* dirs,
+                     const float* scale, float* phase, void* stream);
+
+/* DWI signal: out[2w] = sum_i cos(phase[w,i]), out[2w+1] = sum_i sin(phase[w,i])
+ * with phase as in nz_project_phase, accumulated in FP64.  The reference has no
+ * counterpart (its caller is missing); this is the signal step that turns
+ * DwiSim3d phases into per-direction signals.  out: device double [2W],
+ * overwritten. */
+int nz_signal_reduce(int64_t n, int32_t W, const float* mom, const float* dirs,
+                     const float* scale, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NOZOMI_HIP_H */

@@ -1,0 +1,65 @@
+// Internal types shared by the HIP kernels and the host side of
+// libnozomi_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/nozomi_hip.h"
+
+namespace nz {
+
+// Tolerance band and "no hit" value of the reference walker
+// (sim3d_kernel.cu:34-35) and its sub-step cap (:373).
+constexpr float kTol = 1e-4f;
+constexpr float kLarge = 1e6f;
+constexpr int kMaxIter = 100;
+
+// Device view of a substrate.  The box [-L/2, L/2] per axis is tiled by
+// ncell[a] cells of edge h[a]; one halo cell on each side (local index 0 and
+// ncell+1) holds the explicit image spheres beyond the faces.  A cell's list
+// holds every sphere whose AABB comes within `margin` (+ tol slack) of the
+// cell box; a sub-step whose segment stays inside the cell box grown by
+// `margin` therefore sees every sphere it can touch.
+struct Grid {
+  int32_t ncell[3];    // cells inside the box per axis
+  int32_t nloc[3];     // ncell + 2 (with halo)
+  float h[3];          // cell edge
+  float inv_h[3];
+  float L[3];          // box edge
+  float margin;        // safety margin of the lists
+  int32_t nstruct;     // number of structures; the box has id nstruct
+  int64_t nspheres;
+  // CSR candidate lists over the nloc^3 local cells, cell-relative coords:
+  //   a = (cx - lo_x, cy - lo_y, cz - lo_z, sid)   b = (r^2, (r-tol)^2, (r+tol)^2, r)
+  const int32_t* cell_start;  // [ncells + 1]
+  const float4* cand_a;
+  const float4* cand_b;
+  // every sphere, absolute, for the rare sub-step that leaves the grid
+  const double4* sph;         // (cx, cy, cz, r)
+  const int32_t* sph_sid;
+  // per-structure parameters, box last
+  const float* D;
+  const float* T2;
+};
+
+struct Substrate {
+  Grid g;
+  nz_substrate_info_t info;
+  std::vector<void*> allocations;
+  int device = 0;
+};
+
+// thread-local error message
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+}  // namespace nz
+
+#define NZ_HIP(call)                                           \
+  do {                                                         \
+    hipError_t _e = (call);                                    \
+    if (_e != hipSuccess) return ::nz::hip_fail(_e, #call);    \
+  } while (0)

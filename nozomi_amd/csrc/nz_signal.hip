@@ -1,0 +1,128 @@
+// Phase projection and DWI signal reduction.
+//
+// The reference accumulates one phase per (direction, spin) at every step
+// (randomWalk3d_phase_multidirr, sim3d_kernel.cu:880-942): 8*W bytes of
+// read-modify-write per spin-step.  All of its directions share one gradient
+// waveform G_t per launch, so phase[w] = u_w . sum_t G_t x_t (+ the same
+// projection of the wrap corrections): the walker keeps that 3-vector moment
+// and the W phases are formed once, here.  The signal step (sum over spins of
+// exp(i*phase)) has no counterpart in the reference, whose DWI caller is
+// missing; the sums are accumulated in FP64.
+#include <hip/hip_runtime.h>
+
+#include "nz_internal.h"
+
+namespace nz {
+
+__global__ void __launch_bounds__(256) project_kernel(int64_t n, int32_t W, const float* mom,
+                                                      const float* dirs, const float* scale,
+                                                      float* phase) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.y;
+  if (i >= n) return;
+  const float s = scale ? scale[w] : 1.0f;
+  const float ux = dirs[w], uy = dirs[W + w], uz = dirs[2 * W + w];
+  phase[(int64_t)w * n + i] = s * (ux * mom[i] + uy * mom[n + i] + uz * mom[2 * n + i]);
+}
+
+constexpr int kSigTile = 256;   // spins staged per LDS tile
+constexpr int kSigTiles = 8;    // tiles per block
+
+// Block: kSigTiles*kSigTile spins.  Thread t handles signals t, t+256, ...
+// reading every staged moment from LDS (broadcast reads).
+__global__ void __launch_bounds__(256) signal_kernel(int64_t n, int32_t W, const float* mom,
+                                                     const float* dirs, const float* scale,
+                                                     double* out) {
+  __shared__ float sm[3][kSigTile];
+  const int64_t base = (int64_t)blockIdx.x * kSigTile * kSigTiles;
+  constexpr int kMaxPer = 8;  // signals per thread (W <= 2048)
+  double ac[kMaxPer], as[kMaxPer];
+  float ux[kMaxPer], uy[kMaxPer], uz[kMaxPer];
+#pragma unroll
+  for (int q = 0; q < kMaxPer; ++q) {
+    ac[q] = 0.0;
+    as[q] = 0.0;
+    const int w = threadIdx.x + q * 256;
+    const float s = (w < W) ? (scale ? scale[w] : 1.0f) : 0.0f;
+    ux[q] = (w < W) ? s * dirs[w] : 0.0f;
+    uy[q] = (w < W) ? s * dirs[W + w] : 0.0f;
+    uz[q] = (w < W) ? s * dirs[2 * W + w] : 0.0f;
+  }
+  for (int tile = 0; tile < kSigTiles; ++tile) {
+    const int64_t t0 = base + (int64_t)tile * kSigTile;
+    if (t0 >= n) break;
+    const int64_t i = t0 + threadIdx.x;
+    __syncthreads();
+    sm[0][threadIdx.x] = (i < n) ? mom[i] : 0.0f;
+    sm[1][threadIdx.x] = (i < n) ? mom[n + i] : 0.0f;
+    sm[2][threadIdx.x] = (i < n) ? mom[2 * n + i] : 0.0f;
+    __syncthreads();
+    const int cnt = (int)min<int64_t>(kSigTile, n - t0);
+#pragma unroll
+    for (int q = 0; q < kMaxPer; ++q) {
+      if ((int)threadIdx.x + q * 256 >= W) break;
+      float pc = 0.0f, ps = 0.0f;  // partial sums of 32 spins in FP32, then FP64
+      for (int k = 0; k < cnt; ++k) {
+        const float ph = ux[q] * sm[0][k] + uy[q] * sm[1][k] + uz[q] * sm[2][k];
+        float s, c;
+        sincosf(ph, &s, &c);
+        pc += c;
+        ps += s;
+        if ((k & 31) == 31) {
+          ac[q] += (double)pc;
+          as[q] += (double)ps;
+          pc = 0.0f;
+          ps = 0.0f;
+        }
+      }
+      ac[q] += (double)pc;
+      as[q] += (double)ps;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kMaxPer; ++q) {
+    const int w = threadIdx.x + q * 256;
+    if (w < W) {
+      atomicAdd(&out[2 * w], ac[q]);
+      atomicAdd(&out[2 * w + 1], as[q]);
+    }
+  }
+}
+
+__global__ void zero_f64(double* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+}  // namespace nz
+
+using namespace nz;
+
+extern "C" {
+
+int nz_project_phase(int64_t n, int32_t W, const float* mom, const float* dirs, const float* scale,
+                     float* phase, void* stream) {
+  if (n <= 0 || W <= 0 || W > 65535) return fail(-1, "nz_project_phase: need n > 0 and 0 < W <= 65535");
+  if (!mom || !dirs || !phase) return fail(-1, "nz_project_phase: NULL buffer");
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)W);
+  hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, (hipStream_t)stream, n, W, mom, dirs, scale,
+                     phase);
+  NZ_HIP(hipGetLastError());
+  return 0;
+}
+
+int nz_signal_reduce(int64_t n, int32_t W, const float* mom, const float* dirs, const float* scale,
+                     double* out, void* stream) {
+  if (n <= 0 || W <= 0 || W > 2048) return fail(-1, "nz_signal_reduce: need n > 0 and 0 < W <= 2048");
+  if (!mom || !dirs || !out) return fail(-1, "nz_signal_reduce: NULL buffer");
+  // a fill kernel rather than hipMemsetAsync, so the call can be graph-captured
+  hipLaunchKernelGGL(zero_f64, dim3((unsigned)((2 * W + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, out, (int64_t)2 * W);
+  const int64_t per = (int64_t)kSigTile * kSigTiles;
+  hipLaunchKernelGGL(signal_kernel, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0,
+                     (hipStream_t)stream, n, W, mom, dirs, scale, out);
+  NZ_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

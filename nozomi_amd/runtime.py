@@ -1,0 +1,235 @@
+"""Device-side objects of the walker: substrate handle, resident spin state and
+a GPUArray-like view.
+
+Torch (ROCm) only provides device memory and streams here; all arithmetic on
+the walk path runs in libnozomi_hip.so.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import NzSpinState, NzSubstrateInfo, check, lib
+
+_c_vp = ctypes.c_void_p
+
+
+def _require_gpu(device):
+    if not torch.cuda.is_available():
+        raise _lib.NozomiError("nozomi_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device(device if device is not None else "cuda:0")
+
+
+def _stream_ptr(device):
+    return _c_vp(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t):
+    return _c_vp(t.data_ptr()) if t is not None else _c_vp(0)
+
+
+class DeviceArray:
+    """Thin stand-in for pycuda.gpuarray.GPUArray over a torch tensor.
+
+    The reference's tests read device results as ``spins_d[0,:].get()`` and
+    ``((a - b)**2).get()`` (tests/validation/narrow_pulse/test_01...py:40-52);
+    this class offers .get(), slicing, elementwise arithmetic and .shape."""
+
+    __array_priority__ = 1000
+
+    def __init__(self, tensor):
+        self.tensor = tensor
+
+    def get(self):
+        torch.cuda.synchronize(self.tensor.device)
+        return self.tensor.detach().cpu().numpy()
+
+    @property
+    def shape(self):
+        return tuple(self.tensor.shape)
+
+    @property
+    def dtype(self):
+        return np.dtype(str(self.tensor.dtype).replace("torch.", ""))
+
+    @property
+    def size(self):
+        return self.tensor.numel()
+
+    def __len__(self):
+        return self.tensor.shape[0]
+
+    def __getitem__(self, idx):
+        return DeviceArray(self.tensor[idx])
+
+    def copy(self):
+        return DeviceArray(self.tensor.clone())
+
+    @staticmethod
+    def _v(o):
+        return o.tensor if isinstance(o, DeviceArray) else o
+
+    def __add__(self, o): return DeviceArray(self.tensor + self._v(o))
+    def __radd__(self, o): return DeviceArray(self._v(o) + self.tensor)
+    def __sub__(self, o): return DeviceArray(self.tensor - self._v(o))
+    def __rsub__(self, o): return DeviceArray(self._v(o) - self.tensor)
+    def __mul__(self, o): return DeviceArray(self.tensor * self._v(o))
+    def __rmul__(self, o): return DeviceArray(self._v(o) * self.tensor)
+    def __truediv__(self, o): return DeviceArray(self.tensor / self._v(o))
+    def __rtruediv__(self, o): return DeviceArray(self._v(o) / self.tensor)
+    def __pow__(self, o): return DeviceArray(self.tensor ** self._v(o))
+    def __neg__(self): return DeviceArray(-self.tensor)
+
+    def __repr__(self):
+        return f"DeviceArray({self.shape}, {self.tensor.dtype}, {self.tensor.device})"
+
+
+class DeviceSubstrate:
+    """Spheres + cell grid resident on the GPU (nz_substrate_create).
+
+    spheres: [5, S] (cx, cy, cz, r, sid) float64; D, T2: nstructures + 1
+    values with the box last; L: (Lx, Ly, Lz)."""
+
+    def __init__(self, spheres, nstructures, D, T2, L, cell_size=0.0, margin=0.0, device=None):
+        self.device = _require_gpu(device)
+        sp = np.ascontiguousarray(spheres, dtype=np.float64)
+        if sp.ndim != 2 or sp.shape[0] != 5:
+            raise ValueError("spheres must be a [5, S] array")
+        D = np.ascontiguousarray(D, dtype=np.float32)
+        T2 = np.ascontiguousarray(T2, dtype=np.float32)
+        if D.size != nstructures + 1 or T2.size != nstructures + 1:
+            raise ValueError("D and T2 need nstructures + 1 values (box last)")
+        self.nstructures = int(nstructures)
+        self.L = tuple(float(v) for v in L)
+        self.spheres = sp
+        h = _c_vp()
+        with torch.cuda.device(self.device):
+            check(lib().nz_substrate_create(sp.ctypes.data_as(_c_vp), sp.shape[1], self.nstructures,
+                                            D.ctypes.data_as(_c_vp), T2.ctypes.data_as(_c_vp),
+                                            self.L[0], self.L[1], self.L[2], float(cell_size),
+                                            float(margin), ctypes.byref(h)), "nz_substrate_create")
+        self.handle = h
+
+    def info(self):
+        inf = NzSubstrateInfo()
+        check(lib().nz_substrate_info(self.handle, ctypes.byref(inf)), "nz_substrate_info")
+        return {
+            "nspheres": inf.nspheres, "nstructures": inf.nstructures,
+            "ncell": tuple(inf.ncell), "cell": tuple(inf.cell), "margin": inf.margin,
+            "ncand_total": inf.ncand_total, "ncand_max": inf.ncand_max,
+            "ncand_mean_box": inf.ncand_mean_box, "device_bytes": inf.device_bytes,
+        }
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().nz_substrate_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def seed(self, n, targets=None, seed=0, max_attempts=4096, out=None):
+        """Rejection-seed n spins into the target structures; returns [3, n]."""
+        pos = out if out is not None else torch.empty((3, n), dtype=torch.float32, device=self.device)
+        failed = torch.zeros(1, dtype=torch.int32, device=self.device)
+        t = None if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
+        with torch.cuda.device(self.device):
+            check(lib().nz_seed(self.handle, n, None if t is None else t.ctypes.data_as(_c_vp),
+                                0 if t is None else t.size, int(seed) & (2**64 - 1),
+                                int(max_attempts), _ptr(pos), _ptr(failed),
+                                _stream_ptr(self.device)), "nz_seed")
+        nf = int(failed.item())
+        if nf:
+            raise RuntimeError(f"{nf} spins could not be seeded in {max_attempts} attempts: "
+                               "the target structures are empty or too small")
+        return pos
+
+
+class SpinState:
+    """Caller-owned resident state of n spins (struct nz_spin_state)."""
+
+    def __init__(self, n, device, with_phase=True, with_nsub=False):
+        f32, i32 = torch.float32, torch.int32
+        self.n = int(n)
+        self.device = device
+        self.pos = torch.empty((3, n), dtype=f32, device=device)
+        self.pos0 = torch.empty((3, n), dtype=f32, device=device)
+        self.sig = torch.ones((1, n), dtype=f32, device=device)
+        self.ucell = torch.empty((3, n), dtype=i32, device=device)
+        self.rel = torch.empty((3, n), dtype=f32, device=device)
+        self.ucell0 = torch.empty((3, n), dtype=i32, device=device)
+        self.rel0 = torch.empty((3, n), dtype=f32, device=device)
+        self.logsig = torch.zeros((n,), dtype=f32, device=device)
+        self.sid = torch.empty((n,), dtype=i32, device=device)
+        self.mom = torch.zeros((3, n), dtype=f32, device=device) if with_phase else None
+        self.nsub = torch.zeros((n,), dtype=i32, device=device) if with_nsub else None
+        self.step = 0  # global index of the next step (Philox counter word 1)
+
+    def c_struct(self):
+        return NzSpinState(self.n, _ptr(self.pos), _ptr(self.pos0), _ptr(self.sig),
+                           _ptr(self.ucell), _ptr(self.rel), _ptr(self.ucell0), _ptr(self.rel0),
+                           _ptr(self.logsig), _ptr(self.sid), _ptr(self.mom), _ptr(self.nsub))
+
+    def bytes(self):
+        return sum(t.numel() * t.element_size() for t in (
+            self.pos, self.pos0, self.sig, self.ucell, self.rel, self.ucell0, self.rel0,
+            self.logsig, self.sid, self.mom, self.nsub) if t is not None)
+
+
+def state_init(sub, st):
+    with torch.cuda.device(sub.device):
+        cs = st.c_struct()
+        check(lib().nz_state_init(sub.handle, ctypes.byref(cs), _stream_ptr(sub.device)),
+              "nz_state_init")
+    st.step = 0
+
+
+def walk(sub, st, seed, nsteps, dt, g=None, garea=None, stepmom=None, t2=True):
+    """Advance st by nsteps (nz_walk).  g/garea: float32 device tensors [nsteps]."""
+    if g is not None:
+        if g.dtype != torch.float32 or garea.dtype != torch.float32:
+            raise ValueError("g and garea must be float32 tensors")
+        if g.numel() < nsteps or garea.numel() < nsteps:
+            raise ValueError("g and garea need nsteps entries")
+    if stepmom is not None and (stepmom.dtype != torch.float64 or stepmom.numel() < 6 * nsteps):
+        raise ValueError("stepmom must be a float64 tensor with 6*nsteps entries")
+    with torch.cuda.device(sub.device):
+        cs = st.c_struct()
+        check(lib().nz_walk(sub.handle, ctypes.byref(cs), int(seed) & (2**64 - 1), st.step,
+                            int(nsteps), float(dt), _ptr(g), _ptr(garea), _ptr(stepmom),
+                            _lib.NZ_WALK_T2 if t2 else 0, _stream_ptr(sub.device)), "nz_walk")
+    st.step += int(nsteps)
+
+
+def displacement_moments(sub, st, out=None):
+    out = out if out is not None else torch.empty(6, dtype=torch.float64, device=sub.device)
+    with torch.cuda.device(sub.device):
+        cs = st.c_struct()
+        check(lib().nz_displacement_moments(sub.handle, ctypes.byref(cs), _ptr(out),
+                                            _stream_ptr(sub.device)), "nz_displacement_moments")
+    return out
+
+
+def project_phase(mom, dirs, scale=None, out=None):
+    """phase[w, i] = scale[w] * dirs[:, w] . mom[:, i]  (nz_project_phase)."""
+    n = mom.shape[1]
+    W = dirs.shape[1]
+    out = out if out is not None else torch.empty((W, n), dtype=torch.float32, device=mom.device)
+    check(lib().nz_project_phase(n, W, _ptr(mom), _ptr(dirs.contiguous()),
+                                 _ptr(scale), _ptr(out), _stream_ptr(mom.device)),
+          "nz_project_phase")
+    return out
+
+
+def signal_sums(mom, dirs, scale=None, out=None):
+    """FP64 [W, 2] sums of (cos, sin) of the projected phases (nz_signal_reduce)."""
+    n = mom.shape[1]
+    W = dirs.shape[1]
+    out = out if out is not None else torch.empty((W, 2), dtype=torch.float64, device=mom.device)
+    check(lib().nz_signal_reduce(n, W, _ptr(mom), _ptr(dirs.contiguous()), _ptr(scale),
+                                 _ptr(out), _stream_ptr(mom.device)), "nz_signal_reduce")
+    return out

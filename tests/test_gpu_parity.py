@@ -1,0 +1,199 @@
+"""HIP walker vs the FP64 oracle on the same substrate, start positions and
+Philox stream (the small-substrate configuration: 16 beaded axons in a 30 um
+periodic cube, dt = 20 us).
+
+Bars: per-spin trajectories within 1e-5 um over 1e3 steps for every spin
+whose branch history (number of scatter sub-steps) matches the oracle's;
+spins whose history differs (a floating-point branch flip at a tangent ray or
+a tolerance band) are counted and must stay rare; per-direction signals
+within 1e-3 of the oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from nozomi_amd.protocol import PGSEShells, icosahedron_6
+from nozomi_amd.simulation_engine.diffsim3d import DiffSim3d, DwiSim3d
+from nozomi_amd.substrate import generate_beaded_axons, load_geometry
+
+pytestmark = pytest.mark.gpu
+
+TRAJ_TOL = 1e-5      # um
+SIGNAL_TOL = 1e-3    # absolute
+
+
+@pytest.fixture(scope="module")
+def c0_geometry():
+    M, L = generate_beaded_axons(16, 30.0, seed=20261203)
+    return load_geometry(M, L, D=2.0, T2=100.0)
+
+
+def _oracle_for(geom, oracle_lib):
+    return oracle_lib.OracleSubstrate(geom.spheres, geom.nstructures, geom.structure_D(),
+                                      geom.structure_T2(), (geom.Lx, geom.Ly, geom.Lz))
+
+
+def _min_image(d, L):
+    return d - L * np.round(d / L)
+
+
+def test_seed_matches_oracle(c0_geometry, oracle_lib):
+    g = c0_geometry
+    sub = g.device_substrate()
+    orc = _oracle_for(g, oracle_lib)
+    N = 20000
+    for targets in (list(range(g.nstructures)), [g.nstructures], list(range(g.nstructures + 1))):
+        print(f"seed parity targets={len(targets)}", flush=True)
+        gp = sub.seed(N, targets=targets, seed=99).cpu().numpy().astype(np.float64)
+        op = orc.seed(N, targets, seed=99)
+        same = np.all(gp == op.astype(np.float32).astype(np.float64), axis=0)
+        assert same.mean() > 0.999, f"only {same.mean():.5f} of seeded positions agree"
+        # every GPU position satisfies the acceptance rule
+        inside = np.zeros((g.nstructures + 1, N), bool)
+        for s, st in enumerate(g.structures[:-1]):
+            inside[s] = st.isinside(gp)
+        inside[g.nstructures] = g.isinside(gp)
+        tmask = np.zeros(g.nstructures + 1, bool)
+        tmask[targets] = True
+        first_t = np.where(inside & tmask[:, None], np.arange(g.nstructures + 1)[:, None], 10**9).min(0)
+        first_n = np.where(inside & ~tmask[:, None], np.arange(g.nstructures + 1)[:, None], 10**9).min(0)
+        assert np.mean(first_t < first_n) > 0.9995
+
+
+def test_trajectory_parity_1000_steps(c0_geometry, oracle_lib):
+    g = c0_geometry
+    N, dt, chunk, nchunks = 4096, 0.02, 100, 10
+    sim = DiffSim3d(g, N)
+    sim.setup(initstates_int=1234)
+    sim.state.nsub = torch.zeros(N, dtype=torch.int32, device=sim.device)
+    p_gpu0 = sim.spins_d.get().astype(np.float64)
+    orc = _oracle_for(g, oracle_lib)
+    pos = p_gpu0.copy()
+    pos0 = p_gpu0.copy()
+    sig = np.ones(N)
+    nsub_o = np.zeros(N, dtype=np.int64)
+    L = g.Lx
+    worst = 0.0
+    for c in range(nchunks):
+        print(f"trajectory chunk {c}", flush=True)
+        sim.run(chunk, dt)
+        orc.walk(pos, pos0, sig, seed=1234, step0=c * chunk, nsteps=chunk, dt=dt, nsub=nsub_o)
+        gp = sim.spins_d.get().astype(np.float64)
+        g0 = sim.spins0_d.get().astype(np.float64)
+        nsub_g = sim.state.nsub.cpu().numpy()
+        same_branch = nsub_g == nsub_o
+        err = np.abs(_min_image(gp - pos, L)).max(axis=0)
+        err_disp = np.abs((gp - g0) - (pos - pos0)).max(axis=0)
+        e = np.maximum(err, err_disp)[same_branch]
+        worst = max(worst, float(e.max()))
+        ndiv = int((~same_branch).sum())
+        print(f"  steps={(c + 1) * chunk} max_err={e.max():.3e} divergent={ndiv}/{N}", flush=True)
+        assert e.max() < TRAJ_TOL, f"trajectory error {e.max():.3e} um > {TRAJ_TOL}"
+        assert ndiv <= max(3, int(0.005 * N)), f"{ndiv} spins changed branch history"
+    gs = sim.sig_d.get().ravel()
+    assert np.abs(gs[same_branch] / sig[same_branch] - 1).max() < 1e-5
+    print(f"worst trajectory error over 1000 steps: {worst:.3e} um")
+
+
+def test_phase_signal_parity(c0_geometry, oracle_lib):
+    g = c0_geometry
+    prot = PGSEShells(10.0, 30.0, 0.02, [1.0], icosahedron_6())
+    N = 8192
+    sim = DwiSim3d(g, N, diffdir=prot.dirs)
+    sim.setup(initstates_int=777)
+    p0 = sim.spins_d.get().astype(np.float64)
+
+    class Wave:
+        wave = prot.wave
+        dt = prot.dt
+
+    ph_gpu = sim.simulate_multi_directions(Wave())
+    orc = _oracle_for(g, oracle_lib)
+    pos, pos0, sig = p0.copy(), p0.copy(), np.ones(N)
+    mom = np.zeros((3, N))
+    orc.walk(pos, pos0, sig, seed=777, step0=0, nsteps=prot.nsteps, dt=prot.dt,
+             g=prot.g.astype(np.float64), garea=prot.garea.astype(np.float64), mom=mom)
+    ph_orc = prot.dirs.T @ mom
+    s_gpu = sim.signals()
+    s_orc = np.exp(1j * ph_orc).mean(axis=1)
+    s_gpu_np = np.exp(1j * ph_gpu.astype(np.float64)).mean(axis=1)
+    err = np.abs(s_gpu - s_orc).max()
+    print(f"signal |err| max {err:.3e}; phase err median {np.median(np.abs(ph_gpu - ph_orc)):.3e}")
+    assert err < SIGNAL_TOL
+    assert np.abs(s_gpu - s_gpu_np).max() < 1e-5  # device FP64 reduction == host reduction
+    assert np.median(np.abs(ph_gpu - ph_orc)) < 1e-4
+
+
+def test_free_diffusion_signal():
+    """Empty periodic box: S = exp(-b D) (closed form) for every direction."""
+    from nozomi_amd.simulation_engine.geometry import SimGeometry3D
+    D = 2.0
+    g = SimGeometry3D(30.0, 30.0, 30.0, D, 100.0, 1)
+    prot = PGSEShells(10.0, 30.0, 0.02, [1.0, 2.0], icosahedron_6())
+    N = 200000
+    sim = DwiSim3d(g, N, diffdir=prot.dirs)
+    sim.setup(initstates_int=4242)
+
+    class Wave:
+        wave = prot.wave
+        dt = prot.dt
+
+    sim.simulate_multi_directions(Wave())
+    s = sim.signals(scales=None)
+    expected = np.exp(-prot.b_walk * np.repeat([1.0, 2.0], 6) * D)
+    # std of Re(mean exp(i phi)) <= 1/sqrt(2N)
+    s_sc = sim.signals(scales=prot.scales)
+    tol = 4.0 / np.sqrt(2 * N)
+    print("free diffusion", s_sc.real, expected)
+    assert np.abs(s_sc.real - expected).max() < tol
+    assert np.abs(s_sc.imag).max() < tol
+    assert np.abs(s.real[:6] - expected[:6]).max() < tol
+
+
+def test_inside_single_sphere_confined():
+    """Port of test_inside_single_sphere (test_02...py:17-78), 2e4 spins x 5e3 steps."""
+    from nozomi_amd.substrate import single_sphere_geometry
+    g = single_sphere_geometry(L=10.0, r=1.0, D=3.0)
+    sim = DiffSim3d(g, 20000)
+    sim.set_segments()
+    sim.setup(structures=[0], initstates_int=5)
+    assert g.structures[0].isinside(sim.spins).all()
+    sim.run(5000, 0.002)
+    assert g.structures[0].isinside(sim.spins_d.get()).all()
+    # long-time MSD per axis inside a sphere -> 2 r^2 / 5 (x - x0 over a uniform start)
+    msd = ((sim.spins_d - sim.spins0_d) ** 2).get().mean(axis=1)
+    assert np.all(np.abs(msd / (2 * 1.0 ** 2 / 5) - 1) < 0.05), msd
+
+
+def test_extra_single_sphere_t2():
+    """Port of the extra-axonal single-sphere test (test_02...py:81-153):
+    spins stay outside and mean(sig) follows exp(-t/T2)."""
+    from nozomi_amd.substrate import single_sphere_geometry
+    g = single_sphere_geometry(L=10.0, r=1.0, D=3.0, T2=100.0)
+    sim = DiffSim3d(g, 20000)
+    sim.setup(structures=[1], initstates_int=6)
+    assert not g.structures[0].isinside(sim.spins).any()
+    nt, dt = 5000, 0.002
+    sim.run(nt, dt)
+    p = sim.spins_d.get()
+    assert not g.structures[0].isinside(p).any()
+    assert (np.abs(p) <= 5.0).all()
+    assert np.isclose(sim.sig_d.get().mean(), np.exp(-nt * dt / 100.0), atol=1e-2)
+
+
+def test_free_diffusion_empty_arena_step_api():
+    """Port of test_free_diffusion_empty_arena (test_01...py:13-52) on the
+    reference's one-launch-per-step API, 1e4 spins x 2000 steps."""
+    from nozomi_amd.simulation_engine.geometry import SimGeometry3D
+    Lx = Ly = Lz = 10
+    D, dt, nt, spins = 3.0, 0.002, 2000, 10000
+    sg3 = SimGeometry3D(Lx, Ly, Lz, D, 100, 1)
+    sim = DiffSim3d(sg3, spins)
+    sim.set_segments()
+    for _ in range(nt):
+        sim.step(dt)
+    for a, La in enumerate((Lx, Ly, Lz)):
+        assert (-La / 2 <= sim.spins_d[a, :].get()).all()
+        assert (sim.spins_d[a, :].get() <= La / 2).all()
+    msd = np.sum(((sim.spins_d[0, :] - sim.spins0_d[0, :]) ** 2).get() / spins)
+    assert np.isclose(np.abs(msd / 2 / dt / nt), D, rtol=1e-1)
