@@ -47,13 +47,14 @@ typedef struct nz_spin_state {
   float* pos0;      /* [3,n] start position shifted by every wrap (um)      */
   float* sig;       /* [n]   T2 signal weight                               */
   int32_t* ucell;   /* [3,n] unwrapped cell index per axis                  */
-  float* rel;       /* [3,n] position relative to the cell's lower corner   */
+  double* rel;      /* [3,n] position relative to the cell's lower corner   */
   int32_t* ucell0;  /* [3,n] start cell (unwrapped, in [0, ncell_axis))     */
-  float* rel0;      /* [3,n] start position relative to that cell           */
+  double* rel0;     /* [3,n] start position relative to that cell           */
   float* logsig;    /* [n]   log of sig                                     */
   int32_t* sid;     /* [n]   structure id of the spin after the last step   */
   float* mom;       /* [3,n] phase moment sum_t g_t x_t (may be NULL)       */
   int32_t* nsub;    /* [n]   scatter sub-steps so far (may be NULL)         */
+  int32_t* nguard;  /* [n]   candidates re-solved in FP64 so far (may be NULL) */
 } nz_spin_state;
 
 /* Grid and substrate figures, filled by nz_substrate_info. */
@@ -84,7 +85,7 @@ const char* nz_last_error_string(void);
  *   margin   : per-cell list margin (um, 0 = auto from the jump length)
  * Synchronous (builds on the host, copies to the current device). */
 int nz_substrate_create(const double* spheres_host, int64_t nspheres, int32_t nstructures,
-                        const float* D_host, const float* T2_host, double Lx, double Ly,
+                        const double* D_host, const double* T2_host, double Lx, double Ly,
                         double Lz, double cell_size, double margin, nz_substrate_t* out);
 int nz_substrate_destroy(nz_substrate_t sub);
 int nz_substrate_info(nz_substrate_t sub, nz_substrate_info_t* info);
@@ -126,7 +127,7 @@ int nz_state_init(nz_substrate_t sub, const nz_spin_state* st, void* stream);
  *   flags  : NZ_WALK_* bits. */
 #define NZ_WALK_T2 1u          /* update logsig / sig (randomWalk3d's T2 weight) */
 int nz_walk(nz_substrate_t sub, const nz_spin_state* st, uint64_t seed, int64_t step0,
-            int32_t nsteps, float dt, const float* g, const float* garea, double* stepmom,
+            int32_t nsteps, double dt, const float* g, const float* garea, double* stepmom,
             uint32_t flags, void* stream);
 
 /* Sums over spins of dx^2, dy^2, dz^2, dx^4, dy^4, dz^4 of the current

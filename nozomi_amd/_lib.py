@@ -23,6 +23,7 @@ class NzSpinState(ctypes.Structure):
         ("pos", _c_vp), ("pos0", _c_vp), ("sig", _c_vp),
         ("ucell", _c_vp), ("rel", _c_vp), ("ucell0", _c_vp), ("rel0", _c_vp),
         ("logsig", _c_vp), ("sid", _c_vp), ("mom", _c_vp), ("nsub", _c_vp),
+        ("nguard", _c_vp),
     ]
 
 
@@ -62,7 +63,7 @@ _SIGS = {
                                ctypes.c_int32, _c_vp, _c_vp, _c_vp]),
     "nz_state_init": (ctypes.c_int, [_c_vp, ctypes.POINTER(NzSpinState), _c_vp]),
     "nz_walk": (ctypes.c_int, [_c_vp, ctypes.POINTER(NzSpinState), ctypes.c_uint64,
-                               ctypes.c_int64, ctypes.c_int32, ctypes.c_float, _c_vp, _c_vp,
+                               ctypes.c_int64, ctypes.c_int32, ctypes.c_double, _c_vp, _c_vp,
                                _c_vp, ctypes.c_uint32, _c_vp]),
     "nz_project_phase": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_vp, _c_vp, _c_vp,
                                         _c_vp, _c_vp]),

@@ -1,0 +1,179 @@
+// Host-side construction of the walker's cell grid (used by
+// nz_substrate_create and by the CPU build of the walker in the tests).
+//
+// The box [-L/2, L/2] per axis is tiled exactly by ncell cells; one halo cell
+// per face holds the explicit image spheres beyond it.  A cell's list keeps
+// every sphere whose AABB comes within margin (+ slack for the tolerance band
+// and coordinate rounding) of the cell box, with coordinates relative to the
+// cell's lower corner.  Compare the reference's per-segment lists with a 1 um
+// AABB margin (diffsim3d.py:107-174).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nz_internal.h"
+
+namespace nz {
+
+struct HostGrid {
+  Grid g;  // scalar fields set; pointers bound by bind_host() or by the upload
+  std::vector<int32_t> start;
+  std::vector<float4> ca, cb;
+  std::vector<double4> sph;
+  std::vector<int32_t> sid;
+  std::vector<double> D, T2;
+  nz_substrate_info_t info;
+
+  void bind_host() {
+    g.cell_start = start.data();
+    g.cand_a = ca.data();
+    g.cand_b = cb.data();
+    g.sph = sph.data();
+    g.sph_sid = sid.data();
+    g.D = D.data();
+    g.T2 = T2.data();
+  }
+};
+
+// Returns 0, or -1 with a message in err.
+inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, const double* D,
+                           const double* T2, const double L[3], double cell_size, double margin,
+                           HostGrid& out, std::string& err) {
+  if (S < 0 || (S > 0 && !spheres)) { err = "bad sphere array"; return -1; }
+  if (S > (int64_t)INT32_MAX) { err = "more than 2^31 spheres"; return -1; }
+  if (nstruct < 0 || !D || !T2) { err = "bad structure arrays"; return -1; }
+  for (int a = 0; a < 3; ++a)
+    if (!(L[a] > 0) || !std::isfinite(L[a])) { err = "box edges must be > 0"; return -1; }
+  for (int k = 0; k <= nstruct; ++k)
+    if (!(D[k] >= 0) || !(T2[k] > 0)) { err = "need D >= 0 and T2 > 0"; return -1; }
+  for (int64_t i = 0; i < S; ++i) {
+    const double sd = spheres[4 * S + i];
+    if (!(spheres[3 * S + i] > 0) || sd < 0 || sd >= nstruct || sd != std::floor(sd)) {
+      err = "sphere " + std::to_string(i) + " needs r > 0 and an integer sid in [0, nstructures)";
+      return -1;
+    }
+  }
+  if (cell_size <= 0) cell_size = 2.0;
+  if (margin <= 0) margin = 1.0;
+  Grid& g = out.g;
+  std::memset(&g, 0, sizeof(g));
+  std::memset(&out.info, 0, sizeof(out.info));
+  g.nstruct = nstruct;
+  g.nspheres = S;
+  g.margin = (float)margin;
+  int64_t ncells = 1;
+  for (int a = 0; a < 3; ++a) {
+    g.ncell[a] = std::max(1, (int)std::lround(L[a] / cell_size));
+    if (g.ncell[a] > 1000) { err = "more than 1000 cells per axis; raise cell_size"; return -1; }
+    g.nloc[a] = g.ncell[a] + 2;
+    g.h[a] = (float)(L[a] / g.ncell[a]);
+    g.inv_h[a] = (float)(g.ncell[a] / L[a]);
+    g.L[a] = (float)L[a];
+    g.hd[a] = L[a] / g.ncell[a];
+    g.Ld[a] = L[a];
+    ncells *= g.nloc[a];
+  }
+  const double hd[3] = {L[0] / g.ncell[0], L[1] / g.ncell[1], L[2] / g.ncell[2]};
+  // list margin: the safety margin plus slack for the tolerance band and for
+  // rounding of the cell-relative coordinates
+  const double M = margin + 4.0 * (double)kTol + 1e-5;
+  const double* cx = spheres;
+  const double* cy = spheres + S;
+  const double* cz = spheres + 2 * S;
+  const double* r = spheres + 3 * S;
+  const double* sd = spheres + 4 * S;
+  auto cell_lo = [&](int a, int l) { return -L[a] / 2 + (l - 1) * hd[a]; };
+  auto axis_range = [&](int64_t i, int a, int& lo, int& hi) {
+    const double c = (a == 0 ? cx[i] : a == 1 ? cy[i] : cz[i]);
+    lo = (int)std::floor((c - r[i] - M + L[a] / 2) / hd[a]) - 1;
+    hi = (int)std::floor((c + r[i] + M + L[a] / 2) / hd[a]) + 2;
+    lo = std::max(lo, 0);
+    hi = std::min(hi, g.nloc[a] - 1);
+    while (lo <= hi && cell_lo(a, lo) + hd[a] + M < c - r[i]) ++lo;  // exact AABB test
+    while (hi >= lo && cell_lo(a, hi) - M > c + r[i]) --hi;
+  };
+  std::vector<int64_t> cnt(ncells + 1, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<int64_t> fill;
+    if (pass == 1) {
+      for (int64_t c = 0; c < ncells; ++c) cnt[c + 1] += cnt[c];
+      if (cnt[ncells] > (int64_t)INT32_MAX) {
+        err = "candidate lists exceed 2^31 entries; lower cell_size or margin";
+        return -1;
+      }
+      out.ca.assign((size_t)cnt[ncells], make_float4(0, 0, 0, 0));
+      out.cb.assign((size_t)cnt[ncells], make_float4(0, 0, 0, 0));
+      fill.assign(cnt.begin(), cnt.end() - 1);
+    }
+    for (int64_t i = 0; i < S; ++i) {
+      int lo[3], hi[3];
+      bool any = true;
+      for (int a = 0; a < 3; ++a) {
+        axis_range(i, a, lo[a], hi[a]);
+        any = any && lo[a] <= hi[a];
+      }
+      if (!any) continue;
+      const double ri = r[i];
+      const int32_t idx = (int32_t)i;
+      float idx_bits;
+      std::memcpy(&idx_bits, &idx, sizeof(idx_bits));
+      const float4 bv = make_float4((float)(ri * ri),
+                                    (float)((ri - (double)kTol) * (ri - (double)kTol)),
+                                    (float)((ri + (double)kTol) * (ri + (double)kTol)), idx_bits);
+      for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+          for (int x = lo[0]; x <= hi[0]; ++x) {
+            const size_t c = ((size_t)z * g.nloc[1] + y) * g.nloc[0] + x;
+            if (pass == 0) {
+              cnt[c + 1]++;
+            } else {
+              const int64_t k = fill[c]++;
+              out.ca[k] = make_float4((float)(cx[i] - cell_lo(0, x)), (float)(cy[i] - cell_lo(1, y)),
+                                      (float)(cz[i] - cell_lo(2, z)), (float)sd[i]);
+              out.cb[k] = bv;
+            }
+          }
+    }
+  }
+  out.start.resize(ncells + 1);
+  for (int64_t c = 0; c <= ncells; ++c) out.start[c] = (int32_t)cnt[c];
+  out.sph.resize(S);
+  out.sid.resize(S);
+  for (int64_t i = 0; i < S; ++i) {
+    out.sph[i] = make_double4(cx[i], cy[i], cz[i], r[i]);
+    out.sid[i] = (int32_t)sd[i];
+  }
+  out.D.assign(D, D + nstruct + 1);
+  out.T2.assign(T2, T2 + nstruct + 1);
+
+  nz_substrate_info_t& inf = out.info;
+  inf.nspheres = S;
+  inf.nstructures = nstruct;
+  for (int a = 0; a < 3; ++a) {
+    inf.ncell[a] = g.ncell[a];
+    inf.cell[a] = hd[a];
+  }
+  inf.margin = margin;
+  inf.ncand_total = cnt[ncells];
+  int32_t mx = 0;
+  double box_sum = 0;
+  for (int z = 0; z < g.nloc[2]; ++z)
+    for (int y = 0; y < g.nloc[1]; ++y)
+      for (int x = 0; x < g.nloc[0]; ++x) {
+        const size_t c = ((size_t)z * g.nloc[1] + y) * g.nloc[0] + x;
+        const int32_t n = (int32_t)(cnt[c + 1] - cnt[c]);
+        mx = std::max(mx, n);
+        if (x >= 1 && x <= g.ncell[0] && y >= 1 && y <= g.ncell[1] && z >= 1 && z <= g.ncell[2])
+          box_sum += n;
+      }
+  inf.ncand_max = mx;
+  inf.ncand_mean_box = box_sum / ((double)g.ncell[0] * g.ncell[1] * g.ncell[2]);
+  return 0;
+}
+
+}  // namespace nz

@@ -28,20 +28,24 @@ struct Grid {
   float h[3];          // cell edge
   float inv_h[3];
   float L[3];          // box edge
+  double hd[3];        // cell edge (FP64)
+  double Ld[3];        // box edge (FP64)
   float margin;        // safety margin of the lists
   int32_t nstruct;     // number of structures; the box has id nstruct
   int64_t nspheres;
   // CSR candidate lists over the nloc^3 local cells, cell-relative coords:
-  //   a = (cx - lo_x, cy - lo_y, cz - lo_z, sid)   b = (r^2, (r-tol)^2, (r+tol)^2, r)
+  //   a = (cx - lo_x, cy - lo_y, cz - lo_z, sid)
+  //   b = (r^2, (r-tol)^2, (r+tol)^2, sphere index as int bits)
   const int32_t* cell_start;  // [ncells + 1]
   const float4* cand_a;
   const float4* cand_b;
-  // every sphere, absolute, for the rare sub-step that leaves the grid
+  // every sphere, absolute, FP64: the exact re-solves and the rare sub-step
+  // that leaves the grid's lists
   const double4* sph;         // (cx, cy, cz, r)
   const int32_t* sph_sid;
   // per-structure parameters, box last
-  const float* D;
-  const float* T2;
+  const double* D;
+  const double* T2;
 };
 
 struct Substrate {

@@ -1,0 +1,427 @@
+// Per-spin walk logic, shared by the walk kernel (one spin per lane) and by
+// the CPU build of the walker that the test suite checks against the oracle.
+//
+// Algorithm of the reference walker (sim3d_kernel.cu:340-522, with the phase
+// additions of :524-717 and :719-943), restructured for gfx950:
+//  * the candidate scan (the hot loop: every sphere of the cell list, every
+//    sub-step) runs in FP32 on cell-relative coordinates.  It only has to
+//    pick the decisive spheres: the containing sphere with the furthest exit
+//    and the obstacle with the closest entry;
+//  * a candidate whose FP32 inside test or discriminant falls within a guard
+//    band of its decision threshold is re-solved in FP64, and the two winning
+//    roots are always re-solved in FP64 with the reference's formulas, so the
+//    branch taken and the contact point are those of a double-precision walk;
+//  * the spin position (relative to its cell), the jump vector and the jump
+//    bookkeeping are FP64: per sub-step, not per candidate, so they cost
+//    little, and they keep round-off far below the chaotic growth of
+//    perturbations at repeated wall collisions;
+//  * the structure pass and the collision pass are fused into one scan that
+//    assumes the structure id of the previous sub-step and re-runs only when
+//    the id changed, which is exact with respect to the two-pass order;
+//  * every sub-step checks that its segment stays inside the safety box of
+//    its cell list (cell grown by the list margin) and re-bins or falls back
+//    to all spheres when it does not, so no jump can miss a sphere;
+//  * the T2 weight is kept as a log (one FMA per sub-step instead of an exp);
+//  * phase is kept as the 3-vector moment sum_t g_t x_t: every direction of
+//    the reference's multi-direction kernel is a projection of it, so the
+//    per-step cost is independent of the number of directions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "nz_internal.h"
+#include "nz_rng.h"
+
+namespace nz {
+
+// the reference's comparison macros (sim3d_kernel.cu:7-8), NaN behaviour kept
+NZ_HD inline float MINr(float a, float b) { return (a > b) ? b : a; }
+NZ_HD inline float MAXr(float a, float b) { return (a < b) ? b : a; }
+NZ_HD inline double MINd(double a, double b) { return (a > b) ? b : a; }
+NZ_HD inline double MAXd(double a, double b) { return (a < b) ? b : a; }
+
+// Guard bands of the FP32 decisions, relative to the magnitude of the terms
+// that cancel: dd + r^2 for the inside test dd - r^2, and b^2 + 4a(dd + R^2)
+// for the discriminant b^2 - 4a(dd - R^2) (c = dd - R^2 itself is pure
+// round-off for a spin resting on a tolerance shell).  Single-precision
+// round-off is ~1e-7 of those scales, two orders inside the bands.
+constexpr float kGuardInside = 1e-5f;
+constexpr float kGuardDisc = 2e-5f;
+
+// findStepSizeToSphereInsideTol (sim3d_kernel.cu:77-116) in FP64.
+NZ_HD inline double step_inside_d(double dx, double dy, double dz, double r, double jx, double jy,
+                                  double jz) {
+  const double tol = (double)kTol;
+  if ((dx * dx + dy * dy + dz * dz - r * r) > 0) return (double)kLarge;
+  const double a = jx * jx + jy * jy + jz * jz;
+  const double b = 2 * (jx * dx + jy * dy + jz * dz);
+  const double c = dx * dx + dy * dy + dz * dz - (r - tol) * (r - tol);
+  double sq = b * b - 4 * a * c;
+  if (sq < 0) return 0;
+  sq = sqrt(sq);
+  double q = -0.5 * (b + sq);
+  if (b < 0) q = -0.5 * (b - sq);
+  return MAXd(MAXd(q / a, c / q), 0);
+}
+
+// findStepSizeToSphereOutsideTol (sim3d_kernel.cu:118-160) in FP64.
+NZ_HD inline double step_outside_d(double dx, double dy, double dz, double r, double jx, double jy,
+                                   double jz) {
+  const double tol = (double)kTol;
+  const double a = jx * jx + jy * jy + jz * jz;
+  const double b = 2 * (jx * dx + jy * dy + jz * dz);
+  const double c = dx * dx + dy * dy + dz * dz - (r + tol) * (r + tol);
+  double sq = b * b - 4 * a * c;
+  if (sq < 0) return (double)kLarge;
+  sq = sqrt(sq);
+  double q = -0.5 * (b + sq);
+  if (b < 0) q = -0.5 * (b - sq);
+  const double fpos = q / a, fneg = c / q;
+  if ((dx * dx + dy * dy + dz * dz - r * r) < 0) return MAXd(fpos, fneg);
+  if (fpos > 0 && fneg > 0) return MINd(fpos, fneg);
+  if ((fpos < 0 && fneg < 0) || (b > 0)) return (double)kLarge;
+  return 0;
+}
+
+struct ScanAcc {
+  float sid_new;  // smallest id of a containing sphere (box id if none)
+  float fin;      // furthest exit fraction among containing spheres
+  float fout;     // closest entry fraction among lower-id spheres
+  int kin;        // sphere index of fin (-1: none above 0)
+  int kout;       // sphere index of fout (-1: none)
+  bool any_in;    // inside some relevant sphere
+  int nguard;     // candidates re-solved in FP64
+};
+
+// Lane-invariant inputs of a scan: FP32 copies for the candidate loop and
+// FP64 originals for the re-solves.
+struct ScanIn {
+  float p[3], j[3];
+  float a, inv_a, sid;
+  double pd[3], jd[3];
+  double lo[3];  // absolute lower corner of the lane's cell (FP64)
+};
+
+// FP64 re-solve of one candidate (sphere index k): the exact structure and
+// collision tests of the reference for this sphere.
+NZ_HD __noinline__ void visit_exact(const Grid& g, const ScanIn& in, int k, ScanAcc& acc) {
+  const double4 s = g.sph[k];
+  const float sidc = (float)g.sph_sid[k];
+  const double dx = in.pd[0] - (s.x - in.lo[0]);
+  const double dy = in.pd[1] - (s.y - in.lo[1]);
+  const double dz = in.pd[2] - (s.z - in.lo[2]);
+  const bool inside = (dx * dx + dy * dy + dz * dz - s.w * s.w) < 0;
+  if (inside && sidc < acc.sid_new) acc.sid_new = sidc;
+  if (sidc > in.sid) return;
+  if (inside) {
+    const float f = (float)step_inside_d(dx, dy, dz, s.w, in.jd[0], in.jd[1], in.jd[2]);
+    acc.any_in = true;
+    if (acc.fin < f) { acc.fin = f; acc.kin = k; }
+  } else if (sidc < in.sid) {
+    const float f = (float)step_outside_d(dx, dy, dz, s.w, in.jd[0], in.jd[1], in.jd[2]);
+    if (f < acc.fout) { acc.fout = f; acc.kout = k; }
+  }
+  acc.nguard++;
+}
+
+// One candidate sphere: the structure test (sim3d_kernel.cu:380-398) and the
+// collision test (:414-455) fused.  A = (cx, cy, cz, sid) relative to the
+// lane's cell corner, B = (r^2, (r-tol)^2, (r+tol)^2, sphere index bits).
+// The two root formulas are those of findStepSizeToSphereInsideTol /
+// ...OutsideTol (:77-160) evaluated on one shared quadratic.
+NZ_HD inline void visit(const Grid& g, const ScanIn& in, const float4 A, const float4 B,
+                        ScanAcc& acc) {
+  const float dx = in.p[0] - A.x, dy = in.p[1] - A.y, dz = in.p[2] - A.z;
+  const float dd = dx * dx + dy * dy + dz * dz;
+  const float din = dd - B.x;
+  const float b = 2.0f * (in.j[0] * dx + in.j[1] * dy + in.j[2] * dz);
+  const bool inside = din < 0.0f;                              // isInsideSphere < 0
+  const float R2 = inside ? B.y : B.z;
+  const float c = dd - R2;
+  const float disc = b * b - 4.0f * in.a * c;
+  const bool relevant = A.w <= in.sid;                         // :418-419
+  // FP32 cannot decide this candidate: re-solve it in FP64
+  if (fabsf(din) < kGuardInside * (dd + B.x) ||
+      (relevant && fabsf(disc) < kGuardDisc * (b * b + 4.0f * in.a * (dd + R2)))) {
+    int k;
+    memcpy(&k, &B.w, sizeof(k));
+    visit_exact(g, in, k, acc);
+    return;
+  }
+  if (inside && A.w < acc.sid_new) acc.sid_new = A.w;          // :384-396
+  const float sq = sqrtf(fmaxf(disc, 0.0f));
+  const float q = (b < 0.0f) ? -0.5f * (b - sq) : -0.5f * (b + sq);
+  const float fpos = q * in.inv_a;
+  const float fneg = c * nz_rcpf(q);
+  if (relevant && inside) {                                     // :428-440
+    const float f = (disc < 0.0f) ? 0.0f : MAXr(MAXr(fpos, fneg), 0.0f);
+    acc.any_in = true;
+    if (acc.fin < f) {
+      acc.fin = f;
+      memcpy(&acc.kin, &B.w, sizeof(int));
+    }
+  } else if (relevant && A.w < in.sid) {                        // :441-454
+    float f;
+    if (disc < 0.0f) f = kLarge;
+    else if (fpos > 0.0f && fneg > 0.0f) f = MINr(fpos, fneg);
+    else if ((fpos < 0.0f && fneg < 0.0f) || (b > 0.0f)) f = kLarge;
+    else f = 0.0f;
+    if (f < acc.fout) {
+      acc.fout = f;
+      memcpy(&acc.kout, &B.w, sizeof(int));
+    }
+  }
+}
+
+NZ_HD inline int cell_id(const Grid& g, int lx, int ly, int lz) {
+  return (lz * g.nloc[1] + ly) * g.nloc[0] + lx;
+}
+
+NZ_HD inline double cell_lo(const Grid& g, int a, int l) {
+  return -0.5 * g.Ld[a] + (double)(l - 1) * g.hd[a];
+}
+
+// Segment p -> p + jr*j inside the safety box [-m, h+m]^3 of the lane's cell?
+NZ_HD inline bool in_safety(const Grid& g, const double p[3], const double j[3], double jr) {
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double e = p[a] + jr * j[a];
+    const double lo = -(double)g.margin, hi = g.hd[a] + (double)g.margin;
+    ok = ok && p[a] >= lo && p[a] <= hi && e >= lo && e <= hi;
+  }
+  return ok;
+}
+
+// Move the lane's frame to the cell holding p (in-step re-bin).  Returns
+// false when that cell is outside the grid's halo.
+NZ_HD inline bool rebin(const Grid& g, double p[3], int l[3], int u[3]) {
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int k = (int)floor(p[a] / g.hd[a]);
+    if (k != 0) {
+      const int nl = l[a] + k;
+      if (nl < 0 || nl > g.ncell[a] + 1) {
+        ok = false;
+      } else {
+        l[a] = nl;
+        u[a] += k;
+        p[a] -= (double)k * g.hd[a];
+      }
+    }
+  }
+  return ok;
+}
+
+NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, ScanAcc& acc) {
+  for (int k = cs; k < ce; ++k) {
+    const float4 A = g.cand_a[k];
+    const float4 B = g.cand_b[k];
+    visit(g, in, A, B, acc);
+  }
+}
+
+// Every sphere, for the rare sub-step whose segment leaves the grid's lists.
+NZ_HD __noinline__ void scan_all(const Grid& g, const ScanIn& in, ScanAcc& acc) {
+  for (int64_t k = 0; k < g.nspheres; ++k) visit_exact(g, in, (int)k, acc);
+  acc.nguard -= (int)g.nspheres;
+}
+
+// Resident per-spin state held in registers across the steps of a launch.
+struct Lane {
+  int u[3];      // unwrapped cell index
+  int l[3];      // local cell index (1..ncell inside the box, 0 / ncell+1 halo)
+  double p[3];   // position relative to the lower corner of cell l
+  float logsig;  // log T2 weight
+  float m[3];    // phase moment
+  float sid;     // structure id after the last sub-step
+  int nsub;      // scatter sub-steps
+  int nguard;    // FP64 candidate re-solves
+};
+
+NZ_HD inline void lane_load_cells(const Grid& g, Lane& s) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int w = s.u[a] % g.ncell[a];
+    if (w < 0) w += g.ncell[a];
+    s.l[a] = w + 1;
+  }
+}
+
+NZ_HD inline void draw_jump(uint64_t seed, uint32_t spin, uint32_t step, uint32_t draw, double ds,
+                            double j[3]) {
+  double n0, n1, n2;
+  normals3_d(seed, kStreamWalk, spin, step, draw, n0, n1, n2);
+  j[0] = ds * n0;
+  j[1] = ds * n1;
+  j[2] = ds * n2;
+}
+
+// One time step of one spin: sub-step loop (sim3d_kernel.cu:367-487), phase
+// (:674-676, :880-887), re-base to the cell of the end point and periodic
+// wrap with the phase correction (:489-521, :683-715, :891-942).
+NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin, uint32_t step,
+                            double dt, bool do_t2, float G, float Ga, bool active) {
+  double jr = active ? 1.0 : 0.0;  // :364
+  int niter = 0;
+  double ds = 0.0, j[3] = {0.0, 0.0, 0.0};
+  int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
+  int cs = g.cell_start[cid], ce = g.cell_start[cid + 1];
+  while (jr > (double)kTol && niter < kMaxIter) {  // :373
+    niter++;
+    double jc = jr;
+    if (niter == 1) {  // :400-406
+      ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
+      draw_jump(seed, spin, step, 0u, ds, j);
+    }
+    ScanAcc acc;
+    ScanIn in;
+    for (int pass = 0; pass < 3; ++pass) {
+      bool brute = false;
+      if (!in_safety(g, s.p, j, jr)) {
+        if (rebin(g, s.p, s.l, s.u)) {
+          cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
+          cs = g.cell_start[cid];
+          ce = g.cell_start[cid + 1];
+          brute = !in_safety(g, s.p, j, jr);
+        } else {
+          brute = true;
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        in.pd[a] = s.p[a];
+        in.jd[a] = j[a];
+        in.p[a] = (float)s.p[a];
+        in.j[a] = (float)j[a];
+        in.lo[a] = cell_lo(g, a, s.l[a]);
+      }
+      in.a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
+      in.inv_a = 1.0f / in.a;
+      in.sid = s.sid;
+      acc.sid_new = (float)g.nstruct;
+      acc.fin = 0.0f;
+      acc.fout = kLarge;
+      acc.kin = -1;
+      acc.kout = -1;
+      acc.any_in = false;
+      acc.nguard = 0;
+      if (!brute) scan_list(g, cs, ce, in, acc);
+      else scan_all(g, in, acc);
+      s.nguard += acc.nguard;
+      if (acc.sid_new == s.sid) break;
+      s.sid = acc.sid_new;  // structure changed: redo with the new id
+      if (niter == 1) {
+        ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
+        draw_jump(seed, spin, step, 0u, ds, j);
+      }
+    }
+    // the decisive roots in FP64 (the oracle's arithmetic)
+    double fin = 0.0, fout = (double)kLarge;
+    if (acc.kin >= 0) {
+      const double4 c = g.sph[acc.kin];
+      fin = step_inside_d(s.p[0] - (c.x - in.lo[0]), s.p[1] - (c.y - in.lo[1]),
+                          s.p[2] - (c.z - in.lo[2]), c.w, j[0], j[1], j[2]);
+    }
+    if (acc.kout >= 0) {
+      const double4 c = g.sph[acc.kout];
+      fout = step_outside_d(s.p[0] - (c.x - in.lo[0]), s.p[1] - (c.y - in.lo[1]),
+                            s.p[2] - (c.z - in.lo[2]), c.w, j[0], j[1], j[2]);
+    }
+    jc = MINd(jc, acc.any_in ? fin : (double)kLarge);  // :457
+    jc = MINd(jc, fout);                               // :458
+    s.p[0] += jc * j[0];                               // :460-462
+    s.p[1] += jc * j[1];
+    s.p[2] += jc * j[2];
+    const float inv_t2 = (float)(1.0 / g.T2[(int)s.sid]);
+    if (jc < jr) {  // scatter (:464-479)
+      jr = sqrt(MAXd(0.0, jr * jr - jc * jc));
+      if (do_t2) s.logsig = fmaf((float)(-jc * jc * dt), inv_t2, s.logsig);
+      s.nsub++;
+      if (jr > (double)kTol && niter < kMaxIter)  // the draw is only used by a further sub-step
+        draw_jump(seed, spin, step, (uint32_t)niter, ds, j);
+    } else {  // coherent (:480-486)
+      jr -= jc;
+      if (do_t2) s.logsig = fmaf((float)(-jc * dt), inv_t2, s.logsig);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    // phase after the move, before the wrap
+    const double x = cell_lo(g, a, s.l[a]) + s.p[a];
+    s.m[a] = fmaf(G, (float)x, s.m[a]);
+    // re-base to the cell holding the end point
+    int k = (int)floor(s.p[a] / g.hd[a]);
+    double r = s.p[a] - (double)k * g.hd[a];
+    if (r < 0.0) { r += g.hd[a]; --k; }
+    if (r >= g.hd[a]) { r -= g.hd[a]; ++k; }
+    int nl = s.l[a] + k;
+    if (nl == g.ncell[a] + 1 && r == 0.0) { --nl; --k; r = g.hd[a]; }  // x == L/2 stays
+    s.p[a] = r;
+    s.u[a] += k;
+    // wrap: x > L/2  <=>  nl > ncell;  x < -L/2  <=>  nl < 1.  w = net number
+    // of +L wraps; each one shifts the phase by -L*G_area
+    int w = 0;
+    if (nl > g.ncell[a]) w = (nl - 1) / g.ncell[a];
+    else if (nl < 1) w = -((g.ncell[a] - nl) / g.ncell[a]);
+    if (w != 0) {
+      nl -= w * g.ncell[a];
+      s.m[a] = fmaf(-(float)w * g.L[a], Ga, s.m[a]);
+    }
+    s.l[a] = nl;
+  }
+}
+
+// Displacement since the start of the spin (unwrapped), per axis.
+NZ_HD inline double lane_disp(const Grid& g, const Lane& s, const int u0[3], const double p0[3],
+                              int a) {
+  return (double)(s.u[a] - u0[a]) * g.hd[a] + (s.p[a] - p0[a]);
+}
+
+// Absolute position -> (cell, relative) with the structure id of the point.
+NZ_HD inline void lane_init(const Grid& g, const float pos[3], Lane& s) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double x = (double)pos[a];
+    int c = (int)floor((x + 0.5 * g.Ld[a]) / g.hd[a]);
+    c = c < 0 ? 0 : (c > g.ncell[a] - 1 ? g.ncell[a] - 1 : c);
+    s.u[a] = c;
+    s.l[a] = c + 1;
+    s.p[a] = x - cell_lo(g, a, c + 1);
+    s.m[a] = 0.0f;
+  }
+  // the cell list holds every sphere that can contain a point of the cell;
+  // decided in FP64 like the walk's guarded inside test
+  const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
+  float sid = (float)g.nstruct;
+  for (int k = g.cell_start[cid]; k < g.cell_start[cid + 1]; ++k) {
+    int idx;
+    const float4 B = g.cand_b[k];
+    memcpy(&idx, &B.w, sizeof(idx));
+    const double4 c = g.sph[idx];
+    const double dx = s.p[0] - (c.x - cell_lo(g, 0, s.l[0]));
+    const double dy = s.p[1] - (c.y - cell_lo(g, 1, s.l[1]));
+    const double dz = s.p[2] - (c.z - cell_lo(g, 2, s.l[2]));
+    const float sc = (float)g.sph_sid[idx];
+    if ((dx * dx + dy * dy + dz * dz - c.w * c.w) < 0 && sc < sid) sid = sc;
+  }
+  s.sid = sid;
+  s.logsig = 0.0f;
+  s.nsub = 0;
+  s.nguard = 0;
+}
+
+// Wrapped absolute position and the start position shifted by every wrap
+// (the reference's spins / spins0 after its wrap rule).
+NZ_HD inline void lane_outputs(const Grid& g, const Lane& s, const int u0[3], const double p0[3],
+                               float pos[3], float pos0[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int n = g.ncell[a];
+    const int wraps = (s.u[a] >= 0) ? s.u[a] / n : -((-s.u[a] + n - 1) / n);
+    pos[a] = (float)(cell_lo(g, a, s.l[a]) + s.p[a]);
+    pos0[a] = (float)(-0.5 * g.Ld[a] + (double)u0[a] * g.hd[a] + p0[a] - (double)wraps * g.Ld[a]);
+  }
+}
+
+}  // namespace nz

@@ -96,8 +96,8 @@ class DeviceSubstrate:
         sp = np.ascontiguousarray(spheres, dtype=np.float64)
         if sp.ndim != 2 or sp.shape[0] != 5:
             raise ValueError("spheres must be a [5, S] array")
-        D = np.ascontiguousarray(D, dtype=np.float32)
-        T2 = np.ascontiguousarray(T2, dtype=np.float32)
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        T2 = np.ascontiguousarray(T2, dtype=np.float64)
         if D.size != nstructures + 1 or T2.size != nstructures + 1:
             raise ValueError("D and T2 need nstructures + 1 values (box last)")
         self.nstructures = int(nstructures)
@@ -153,31 +153,33 @@ class SpinState:
     """Caller-owned resident state of n spins (struct nz_spin_state)."""
 
     def __init__(self, n, device, with_phase=True, with_nsub=False):
-        f32, i32 = torch.float32, torch.int32
+        f32, f64, i32 = torch.float32, torch.float64, torch.int32
         self.n = int(n)
         self.device = device
         self.pos = torch.empty((3, n), dtype=f32, device=device)
         self.pos0 = torch.empty((3, n), dtype=f32, device=device)
         self.sig = torch.ones((1, n), dtype=f32, device=device)
         self.ucell = torch.empty((3, n), dtype=i32, device=device)
-        self.rel = torch.empty((3, n), dtype=f32, device=device)
+        self.rel = torch.empty((3, n), dtype=f64, device=device)
         self.ucell0 = torch.empty((3, n), dtype=i32, device=device)
-        self.rel0 = torch.empty((3, n), dtype=f32, device=device)
+        self.rel0 = torch.empty((3, n), dtype=f64, device=device)
         self.logsig = torch.zeros((n,), dtype=f32, device=device)
         self.sid = torch.empty((n,), dtype=i32, device=device)
         self.mom = torch.zeros((3, n), dtype=f32, device=device) if with_phase else None
         self.nsub = torch.zeros((n,), dtype=i32, device=device) if with_nsub else None
+        self.nguard = torch.zeros((n,), dtype=i32, device=device) if with_nsub else None
         self.step = 0  # global index of the next step (Philox counter word 1)
 
     def c_struct(self):
         return NzSpinState(self.n, _ptr(self.pos), _ptr(self.pos0), _ptr(self.sig),
                            _ptr(self.ucell), _ptr(self.rel), _ptr(self.ucell0), _ptr(self.rel0),
-                           _ptr(self.logsig), _ptr(self.sid), _ptr(self.mom), _ptr(self.nsub))
+                           _ptr(self.logsig), _ptr(self.sid), _ptr(self.mom), _ptr(self.nsub),
+                           _ptr(self.nguard))
 
     def bytes(self):
         return sum(t.numel() * t.element_size() for t in (
             self.pos, self.pos0, self.sig, self.ucell, self.rel, self.ucell0, self.rel0,
-            self.logsig, self.sid, self.mom, self.nsub) if t is not None)
+            self.logsig, self.sid, self.mom, self.nsub, self.nguard) if t is not None)
 
 
 def state_init(sub, st):
@@ -233,3 +235,16 @@ def signal_sums(mom, dirs, scale=None, out=None):
     check(lib().nz_signal_reduce(n, W, _ptr(mom), _ptr(dirs.contiguous()), _ptr(scale),
                                  _ptr(out), _stream_ptr(mom.device)), "nz_signal_reduce")
     return out
+
+
+def state_f64(sub, st):
+    """FP64 views of the resident state: absolute wrapped positions [3, n] and
+    unwrapped displacements since nz_state_init [3, n] (numpy)."""
+    inf = sub.info()
+    h = torch.tensor(inf["cell"], dtype=torch.float64, device=st.ucell.device)[:, None]
+    n = torch.tensor(inf["ncell"], dtype=torch.int64, device=st.ucell.device)[:, None]
+    L = torch.tensor(sub.L, dtype=torch.float64, device=st.ucell.device)[:, None]
+    u = st.ucell.to(torch.int64)
+    pos = -0.5 * L + torch.remainder(u, n).to(torch.float64) * h + st.rel
+    disp = (u - st.ucell0.to(torch.int64)).to(torch.float64) * h + (st.rel - st.rel0)
+    return pos.cpu().numpy(), disp.cpu().numpy()

@@ -1,0 +1,110 @@
+// CPU build of the walker's per-spin core (nozomi_amd/csrc/nz_walk_core.h and
+// nz_grid_build.h), for tests that check the kernel logic against the FP64
+// oracle without a GPU.  Test infrastructure only.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../nozomi_amd/csrc/nz_grid_build.h"
+#include "../../nozomi_amd/csrc/nz_walk_core.h"
+
+namespace nz {
+// error helpers referenced by nz_internal.h (the library defines them in
+// nz_substrate.hip)
+static std::string g_cpu_err;
+void set_error(const std::string& msg) { g_cpu_err = msg; }
+int fail(int code, const std::string& msg) { set_error(msg); return code; }
+int hip_fail(hipError_t e, const char* what) { set_error(what); return (int)e; }
+}  // namespace nz
+
+using namespace nz;
+
+struct CpuWalker {
+  HostGrid hg;
+  std::vector<Lane> lanes;
+  std::vector<int> u0;
+  std::vector<double> p0;
+};
+
+extern "C" {
+
+void* nzcpu_create(const double* spheres, int64_t S, int32_t nstruct, const double* D,
+                   const double* T2, double Lx, double Ly, double Lz, double cell, double margin) {
+  CpuWalker* w = new CpuWalker();
+  const double L[3] = {Lx, Ly, Lz};
+  std::string err;
+  if (build_grid_host(spheres, S, nstruct, D, T2, L, cell, margin, w->hg, err) != 0) {
+    delete w;
+    return nullptr;
+  }
+  w->hg.bind_host();
+  return w;
+}
+
+void nzcpu_destroy(void* h) { delete (CpuWalker*)h; }
+
+void nzcpu_info(void* h, double* out /* ncell[3], cell[3], margin, ncand_mean_box */) {
+  const nz_substrate_info_t& inf = ((CpuWalker*)h)->hg.info;
+  for (int a = 0; a < 3; ++a) { out[a] = inf.ncell[a]; out[3 + a] = inf.cell[a]; }
+  out[6] = inf.margin;
+  out[7] = inf.ncand_mean_box;
+}
+
+// pos: float [3, N] absolute start positions
+void nzcpu_init(void* h, int N, const float* pos) {
+  CpuWalker* w = (CpuWalker*)h;
+  w->lanes.resize(N);
+  w->u0.resize(3 * (size_t)N);
+  w->p0.resize(3 * (size_t)N);
+  for (int i = 0; i < N; ++i) {
+    const float p[3] = {pos[i], pos[N + i], pos[2 * N + i]};
+    lane_init(w->hg.g, p, w->lanes[i]);
+    for (int a = 0; a < 3; ++a) {
+      w->u0[3 * i + a] = w->lanes[i].u[a];
+      w->p0[3 * i + a] = w->lanes[i].p[a];
+    }
+  }
+}
+
+// Walk every spin nsteps; outputs absolute pos/pos0 [3,N], nsub [N] (total),
+// mom [3,N] (when g given), logsig [N].
+void nzcpu_walk(void* h, uint64_t seed, int64_t step0, int nsteps, double dt, const float* g,
+                const float* garea, int do_t2, float* pos, float* pos0, int32_t* nsub, float* mom,
+                float* logsig, int32_t* nguard) {
+  CpuWalker* w = (CpuWalker*)h;
+  const int N = (int)w->lanes.size();
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < N; ++i) {
+    Lane& s = w->lanes[i];
+    for (int t = 0; t < nsteps; ++t) {
+      const float G = g ? g[t] : 0.0f, Ga = g ? garea[t] : 0.0f;
+      walk_step(w->hg.g, s, seed, (uint32_t)i, (uint32_t)(step0 + t), dt, do_t2 != 0, G, Ga, true);
+    }
+    float pp[3], pp0[3];
+    lane_outputs(w->hg.g, s, &w->u0[3 * i], &w->p0[3 * i], pp, pp0);
+    for (int a = 0; a < 3; ++a) {
+      pos[a * N + i] = pp[a];
+      pos0[a * N + i] = pp0[a];
+      if (mom) mom[a * N + i] = s.m[a];
+    }
+    if (nsub) nsub[i] = s.nsub;
+    if (logsig) logsig[i] = s.logsig;
+    if (nguard) nguard[i] = s.nguard;
+  }
+}
+
+// FP64 absolute positions (wrapped) and unwrapped displacements [3, N].
+void nzcpu_state_f64(void* h, double* pos, double* disp) {
+  CpuWalker* w = (CpuWalker*)h;
+  const Grid& g = w->hg.g;
+  const int N = (int)w->lanes.size();
+  for (int i = 0; i < N; ++i) {
+    const Lane& s = w->lanes[i];
+    for (int a = 0; a < 3; ++a) {
+      pos[a * N + i] = cell_lo(g, a, s.l[a]) + s.p[a];
+      disp[a * N + i] = lane_disp(g, s, &w->u0[3 * i], &w->p0[3 * i], a);
+    }
+  }
+}
+
+}  // extern "C"

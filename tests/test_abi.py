@@ -1,0 +1,39 @@
+"""libnozomi_hip.so loads and exports every entry point include/nozomi_hip.h
+declares (no GPU call)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "nozomi_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(nz_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for n in ("nz_substrate_create", "nz_walk", "nz_seed", "nz_state_init", "nz_signal_reduce"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from nozomi_amd import _lib
+    L = _lib.lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+    assert L.nz_abi_version() == _lib.ABI_VERSION
+    assert set(_lib.EXPORTED) == set(_declared())
+
+
+def test_argument_errors_are_reported():
+    from nozomi_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    rc = L.nz_substrate_create(None, 0, 0, None, None, 10.0, 10.0, 10.0, 0.0, 0.0, ctypes.byref(h))
+    assert rc < 0
+    assert b"structure" in L.nz_last_error_string()
+    rc = L.nz_walk(None, None, 0, 0, 1, 0.01, None, None, None, 0, None)
+    assert rc < 0 and b"substrate" in L.nz_last_error_string()

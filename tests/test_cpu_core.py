@@ -1,0 +1,98 @@
+"""The walk kernel's per-spin logic (CPU build of nz_walk_core.h) against the
+FP64 oracle, on the small-substrate configuration (16 beaded axons in a 30 um
+periodic cube, dt = 20 us), same start positions and Philox stream.
+
+Trajectory agreement is limited by the dynamics, not by arithmetic: at every
+wall collision a perturbation of the contact point is amplified (the end of a
+chord depends on its start through 1/cos of the incidence angle), so any two
+implementations that round differently separate exponentially in the number
+of collisions.  test_oracle_is_chaotic shows this on the FP64 oracle itself;
+the bar for the kernel is therefore set per collision count.
+"""
+import numpy as np
+import pytest
+
+from nozomi_amd.substrate import generate_beaded_axons, load_geometry, single_sphere_geometry
+from nozomi_amd.simulation_engine.geometry import SimGeometry3D
+from tests.cpu_build import CpuWalker
+
+
+def _min_image(d, L):
+    return d - L * np.round(d / L)
+
+
+@pytest.fixture(scope="module")
+def c0(oracle_lib):
+    M, L = generate_beaded_axons(16, 30.0, seed=20261203)
+    g = load_geometry(M, L, D=2.0, T2=100.0)
+    o = oracle_lib.OracleSubstrate(g.spheres, g.nstructures, g.structure_D(), g.structure_T2(), (L, L, L))
+    return g, o
+
+
+def _run_pair(g, o, N, nsteps, dt=0.02, seed=1234, targets=None):
+    L = g.Lx
+    p = o.seed(N, targets if targets is not None else np.arange(g.nstructures + 1), seed=5)
+    p = p.astype(np.float32)
+    cw = CpuWalker(g)
+    cw.init(p)
+    out = cw.walk(seed, 0, nsteps, dt)
+    pos = p.astype(np.float64)
+    pos0 = pos.copy()
+    sig = np.ones(N)
+    nsub = np.zeros(N, np.int64)
+    o.walk(pos, pos0, sig, seed=seed, step0=0, nsteps=nsteps, dt=dt, nsub=nsub)
+    err = np.maximum(np.abs(_min_image(out["pos_f64"] - pos, L)).max(0),
+                     np.abs(out["disp_f64"] - (pos - pos0)).max(0))
+    return out, err, nsub, sig
+
+
+def test_free_diffusion_bitlevel(oracle_lib):
+    g = SimGeometry3D(30.0, 30.0, 30.0, 2.0, 100.0, 1)
+    o = oracle_lib.OracleSubstrate(g.spheres, 0, g.structure_D(), g.structure_T2(), (30.0, 30.0, 30.0))
+    out, err, nsub, _ = _run_pair(g, o, 2048, 200)
+    assert err.max() < 1e-12
+    assert (out["nsub"] == 0).all()
+
+
+def test_single_sphere_short(oracle_lib):
+    g = single_sphere_geometry(L=10.0, r=1.0, D=2.0)
+    o = oracle_lib.OracleSubstrate(g.spheres, 1, g.structure_D(), g.structure_T2(), (10.0, 10.0, 10.0))
+    for targets in ([0], [1]):
+        out, err, nsub, _ = _run_pair(g, o, 2048, 10, targets=targets)
+        assert (out["nsub"] == nsub).all()
+        assert err.max() < 1e-8, err.max()
+
+
+def test_trajectories_1000_steps(c0):
+    g, o = c0
+    out, err, nsub, sig = _run_pair(g, o, 4096, 1000)
+    same = out["nsub"] == nsub
+    few = nsub <= 60
+    print(f"within 1e-5: all {np.mean(err < 1e-5):.4f}; <=60 collisions {np.mean(err[few] < 1e-5):.4f}; "
+          f"branch-history mismatches {np.sum(~same)}; FP64 re-solves {out['nguard'].sum()}")
+    assert np.mean(err[few] < 1e-5) > 0.995
+    assert np.mean(err < 1e-5) > 0.85
+    assert np.median(err[few]) < 1e-9
+    s = np.exp(out["logsig"].astype(np.float64))
+    assert np.abs(s[few] / sig[few] - 1).max() < 1e-5
+
+
+def test_oracle_is_chaotic(c0):
+    """The FP64 oracle against itself with start positions perturbed by
+    1e-9 um: spins with many wall collisions separate beyond 1e-5 um."""
+    g, o = c0
+    N = 2048
+    p = o.seed(N, np.arange(g.nstructures + 1), seed=5)
+    q = p + np.random.default_rng(0).normal(size=p.shape) * 1e-9
+    res = []
+    for start in (p, q):
+        pos = start.copy()
+        pos0 = start.copy()
+        nsub = np.zeros(N, np.int64)
+        o.walk(pos, pos0, np.ones(N), seed=1234, step0=0, nsteps=1000, dt=0.02, nsub=nsub)
+        res.append((pos, nsub))
+    err = np.abs(_min_image(res[0][0] - res[1][0], g.Lx)).max(0)
+    many = res[0][1] > 200
+    assert many.sum() > 20
+    assert np.mean(err[many] > 1e-5) > 0.5        # collision-rich spins diverge
+    assert np.mean(err[res[0][1] <= 10] < 1e-7) > 0.99   # collision-poor spins do not
