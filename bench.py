@@ -7,8 +7,9 @@ position), D = 2.0 um^2/ms everywhere; dt = 10 us, 5000 time steps; PGSE
 little_delta = 10 ms, big_delta = 40 ms at b = 1, 2, 3 ms/um^2 with 64
 Fibonacci-hemisphere directions per shell (192 signals).  Inputs are resident
 in HBM before timing; each timed pass re-initialises the spin state from the
-same start positions (nz_state_init), walks 5000 steps with phase in one
-launch, and reduces the 192 signals in FP64.
+same start positions (nz_walker_set_positions), walks 5000 steps with phase
+(persistent launches between spatial re-sorts) and reduces the 192 signals in
+FP64.
 
 Prints one JSON line (rank 0).  The CPU baseline is the FP64 oracle (OpenMP)
 on a bounded subsample of the same workload on this host's cores.
@@ -39,6 +40,7 @@ def parse():
     ap.add_argument("--box", type=float, default=60.0)
     ap.add_argument("--cell", type=float, default=0.0)
     ap.add_argument("--margin", type=float, default=0.0)
+    ap.add_argument("--sort", type=int, default=32, help="steps between spatial re-sorts")
     ap.add_argument("--cpu-sample", type=int, default=20000,
                     help="spins of the CPU-baseline subsample (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=1000)
@@ -65,26 +67,27 @@ def main():
     N = args.spins
     sub = geom.device_substrate(cell_size=args.cell, margin=args.margin, device=dev)
     info = sub.info()
-    st = rt.SpinState(N, dev, with_phase=True)
+    walker = rt.Walker(sub, N, sort_interval=args.sort)
     start = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=args.seed + 1)
     g = torch.from_numpy(prot.g[:T].copy()).to(dev)
     ga = torch.from_numpy(prot.garea[:T].copy()).to(dev)
     dirs = torch.from_numpy(prot.dirs.astype(np.float32)).to(dev)
     scales = torch.from_numpy(prot.scales.astype(np.float32)).to(dev)
+    mom = torch.empty((3, N), dtype=torch.float32, device=dev)
     sums = torch.empty((prot.nsignals, 2), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
 
     walk_ms = []
 
     def one_pass(record):
-        st.pos.copy_(start)
-        rt.state_init(sub, st)
+        walker.set_positions(start)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        rt.walk(sub, st, args.seed, T, prot.dt, g=g, garea=ga, t2=False)
+        walker.walk(args.seed, T, prot.dt, g=g, garea=ga, t2=False)
         e1.record(stream)
-        rt.signal_sums(st.mom, dirs, scales, out=sums)
+        walker.read(mom=mom)
+        rt.signal_sums(mom, dirs, scales, out=sums)
         if record:
             walk_ms.append((e0, e1))
 
@@ -132,7 +135,7 @@ def main():
             "spins": N, "time_steps": T, "dt_ms": prot.dt, "signals": prot.nsignals,
             "spheres": int(info["nspheres"]), "structures": int(info["nstructures"]),
             "cells": list(info["ncell"]), "cell_um": info["cell"][0], "margin_um": info["margin"],
-            "mean_candidates": cbar, "scatter_per_step": sbar,
+            "mean_candidates": cbar, "scatter_per_step": sbar, "sort_interval": args.sort,
         },
         "walk_kernel_ms": kern_ms,
         "signals_per_s": prot.nsignals * args.steps / elapsed,
@@ -154,14 +157,12 @@ def main():
 
 def measure_sbar(geom, sub, prot, dev, seed, n=65536, steps=200):
     """Mean scatter sub-steps per spin-step on a subsample (GPU counter)."""
-    import torch
-
     from nozomi_amd import runtime as rt
-    st = rt.SpinState(n, dev, with_phase=False, with_nsub=True)
-    sub.seed(n, targets=np.arange(geom.nstructures + 1), seed=seed + 7, out=st.pos)
-    rt.state_init(sub, st)
-    rt.walk(sub, st, seed, steps, prot.dt, t2=False)
-    return float(st.nsub.sum().item()) / (n * steps)
+    w = rt.Walker(sub, n, counters=True)
+    w.seed(targets=np.arange(geom.nstructures + 1), seed=seed + 7)
+    w.walk(seed, steps, prot.dt, t2=False)
+    nsub, _ = w.counters()
+    return float(nsub.sum()) / (n * steps)
 
 
 def cpu_baseline(geom, prot, start, args):

@@ -37,26 +37,6 @@ extern "C" {
 
 typedef struct nz_substrate* nz_substrate_t;
 
-/* Resident per-spin state.  All arrays are caller-owned device buffers.
- * pos/pos0/sig mirror the reference's spins_d, spins0_d and sig_d
- * (diffsim3d.py:43-44,62) and are rewritten by every walk call; the
- * authoritative state between calls is the cell-relative part. */
-typedef struct nz_spin_state {
-  int64_t n;        /* number of spins                                      */
-  float* pos;       /* [3,n] absolute position, wrapped into the box (um)   */
-  float* pos0;      /* [3,n] start position shifted by every wrap (um)      */
-  float* sig;       /* [n]   T2 signal weight                               */
-  int32_t* ucell;   /* [3,n] unwrapped cell index per axis                  */
-  double* rel;      /* [3,n] position relative to the cell's lower corner   */
-  int32_t* ucell0;  /* [3,n] start cell (unwrapped, in [0, ncell_axis))     */
-  double* rel0;     /* [3,n] start position relative to that cell           */
-  float* logsig;    /* [n]   log of sig                                     */
-  int32_t* sid;     /* [n]   structure id of the spin after the last step   */
-  float* mom;       /* [3,n] phase moment sum_t g_t x_t (may be NULL)       */
-  int32_t* nsub;    /* [n]   scatter sub-steps so far (may be NULL)         */
-  int32_t* nguard;  /* [n]   candidates re-solved in FP64 so far (may be NULL) */
-} nz_spin_state;
-
 /* Grid and substrate figures, filled by nz_substrate_info. */
 typedef struct nz_substrate_info_t {
   int64_t nspheres;
@@ -102,40 +82,72 @@ int nz_substrate_info(nz_substrate_t sub, nz_substrate_info_t* info);
 int nz_seed(nz_substrate_t sub, int64_t n, const int32_t* targets_host, int32_t ntargets,
             uint64_t seed, int32_t max_attempts, float* pos, int32_t* nfailed, void* stream);
 
-/* Build the resident state from absolute positions st->pos (device [3,n]):
- * cell-relative coordinates, pos0 = pos, sig = 1, mom = 0, nsub = 0 and the
- * structure id of every spin.  Replaces the state set-up of DiffSim3d.setup
- * (diffsim3d.py:42-45,56-62); initstates (sim3d_kernel.cu:65-70) has no
- * counterpart: the Philox stream is stateless. */
-int nz_state_init(nz_substrate_t sub, const nz_spin_state* st, void* stream);
+/* A walker: the resident state of n spins on the substrate's device.
+ * Replaces the device arrays DiffSim3d.setup allocates (spins_d, spins0_d,
+ * sig_d, phase_d and the cuRAND states; diffsim3d.py:35-63).  The state is
+ * kept in cell-relative double precision, ordered by cell (re-sorted every
+ * sort_interval steps so that a wavefront's spins share candidate lists);
+ * spin i keeps its Philox key i whatever its slot.  Results come back in spin
+ * order through nz_walker_read.  Library-owned; device memory ~210 B/spin. */
+typedef struct nz_walker* nz_walker_t;
+
+#define NZ_WALKER_COUNTERS 1u  /* keep per-spin scatter and FP64-rescan counters */
+
+int nz_walker_create(nz_substrate_t sub, int64_t n, uint32_t flags, nz_walker_t* out);
+int nz_walker_destroy(nz_walker_t w);
+/* Steps between spatial re-sorts of the spins (0 = never sort). */
+int nz_walker_set_sort_interval(nz_walker_t w, int32_t steps);
+/* Global index of the next step (the Philox step counter); set it to resume. */
+int nz_walker_get_step(nz_walker_t w, int64_t* step);
+int nz_walker_set_step(nz_walker_t w, int64_t step);
+
+/* Start positions pos [3,n] (absolute, device): resets the state (pos0 = pos,
+ * sig = 1, phase moment 0, counters 0, step 0) and finds every spin's
+ * structure id.  Replaces the set-up of DiffSim3d.setup (diffsim3d.py:42-62);
+ * initstates (sim3d_kernel.cu:65-70) has no counterpart: the Philox stream is
+ * stateless. */
+int nz_walker_set_positions(nz_walker_t w, const float* pos, void* stream);
+
+/* Rejection-seed the walker's spins into the target structures (as nz_seed)
+ * and set them as start positions.  nfailed: device int32 (may be NULL). */
+int nz_walker_seed(nz_walker_t w, const int32_t* targets_host, int32_t ntargets, uint64_t seed,
+                   int32_t max_attempts, int32_t* nfailed, void* stream);
 
 /* Advance every spin by nsteps time steps of length dt (ms).
  * Replaces nsteps launches of randomWalk3d / randomWalk3d_phase /
  * randomWalk3d_phase_multidirr (sim3d_kernel.cu:340-943, launched by
- * diffsim3d.py:199-226) with one persistent launch.
- *   seed   : Philox key; draw k of step s of spin i uses counter (k, s, i, 0)
- *   step0  : global index of the first step (continues the stream)
- *   g, garea : device [nsteps] per-step phase coefficient and wrap area,
+ * diffsim3d.py:199-226) with persistent launches of many steps each.
+ *   seed   : Philox key; draw k of step s of spin i uses counter (k, s, i, 0),
+ *            s counting from the walker's step counter, which advances
+ *   g, garea : device float [nsteps] per-step phase coefficient and wrap area,
  *              i.e. the G and G_area arguments of the reference launches
  *              (diffsim3d.py:253-255); NULL for a walk without phase.
- *              With them, st->mom accumulates sum_t g_t x_t with the
- *              reference's wrap correction (sim3d_kernel.cu:674-715,880-942).
- *   stepmom: device double [nsteps, 6] or NULL; receives the sums over spins
- *            of dx^2, dy^2, dz^2, dx^4, dy^4, dz^4 after every step (the
+ *              With them the walker's moment accumulates sum_t g_t x_t with
+ *              the reference's wrap correction (sim3d_kernel.cu:674-715,880-942).
+ *   stepmom: device double [nsteps, 6] or NULL; receives (adds) the sums over
+ *            spins of dx^2, dy^2, dz^2, dx^4, dy^4, dz^4 after every step (the
  *            moment sums of computeDiffusionCoefficientsAndKurtosis,
- *            sim3d_kernel.cu:162-238), accumulated with atomics.
+ *            sim3d_kernel.cu:162-238).
  *   flags  : NZ_WALK_* bits. */
-#define NZ_WALK_T2 1u          /* update logsig / sig (randomWalk3d's T2 weight) */
-int nz_walk(nz_substrate_t sub, const nz_spin_state* st, uint64_t seed, int64_t step0,
-            int32_t nsteps, double dt, const float* g, const float* garea, double* stepmom,
-            uint32_t flags, void* stream);
+#define NZ_WALK_T2 1u          /* update the T2 weight (randomWalk3d's sig) */
+int nz_walk(nz_walker_t w, uint64_t seed, int32_t nsteps, double dt, const float* g,
+            const float* garea, double* stepmom, uint32_t flags, void* stream);
+
+/* The reference's views in spin order, any pointer may be NULL:
+ * pos [3,n] wrapped positions (spins_d), pos0 [3,n] start positions shifted by
+ * every wrap (spins0_d), sig [n] T2 weight (sig_d), mom [3,n] phase moment. */
+int nz_walker_read(nz_walker_t w, float* pos, float* pos0, float* sig, float* mom, void* stream);
+/* FP64 wrapped positions and unwrapped displacements [3,n] in spin order. */
+int nz_walker_read_f64(nz_walker_t w, double* pos, double* disp, void* stream);
+/* Per-spin counters [n] (walker created with NZ_WALKER_COUNTERS): scatter
+ * sub-steps and sub-steps whose candidate scan ran in FP64. */
+int nz_walker_counters(nz_walker_t w, int32_t* nsub, int32_t* nguard, void* stream);
 
 /* Sums over spins of dx^2, dy^2, dz^2, dx^4, dy^4, dz^4 of the current
- * displacement (position minus spins0, i.e. the unwrapped displacement since
- * nz_state_init): the block sums of computeDiffusionCoefficientsAndKurtosis
- * (sim3d_kernel.cu:162-238) before the host scaling D = S/(2tN), K = S/N.
- * out: device double [6], overwritten. */
-int nz_displacement_moments(nz_substrate_t sub, const nz_spin_state* st, double* out, void* stream);
+ * displacement (position minus spins0): the block sums of
+ * computeDiffusionCoefficientsAndKurtosis (sim3d_kernel.cu:162-238) before the
+ * host scaling D = S/(2tN), K = S/N.  out: device double [6], overwritten. */
+int nz_displacement_moments(nz_walker_t w, double* out, void* stream);
 
 /* Phases of W directions from the moments: phase[w, i] = scale[w] *
  * (dirs[0,w] m_x + dirs[1,w] m_y + dirs[2,w] m_z).  Replaces the per-step

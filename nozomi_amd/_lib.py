@@ -12,19 +12,9 @@ LIB_PATH = os.path.join(_HERE, "lib", "libnozomi_hip.so")
 ABI_VERSION = 1
 
 NZ_WALK_T2 = 1
+NZ_WALKER_COUNTERS = 1
 
 _c_vp = ctypes.c_void_p
-
-
-class NzSpinState(ctypes.Structure):
-    """struct nz_spin_state (include/nozomi_hip.h)."""
-    _fields_ = [
-        ("n", ctypes.c_int64),
-        ("pos", _c_vp), ("pos0", _c_vp), ("sig", _c_vp),
-        ("ucell", _c_vp), ("rel", _c_vp), ("ucell0", _c_vp), ("rel0", _c_vp),
-        ("logsig", _c_vp), ("sid", _c_vp), ("mom", _c_vp), ("nsub", _c_vp),
-        ("nguard", _c_vp),
-    ]
 
 
 class NzSubstrateInfo(ctypes.Structure):
@@ -61,16 +51,25 @@ _SIGS = {
     "nz_substrate_info": (ctypes.c_int, [_c_vp, ctypes.POINTER(NzSubstrateInfo)]),
     "nz_seed": (ctypes.c_int, [_c_vp, ctypes.c_int64, _c_vp, ctypes.c_int32, ctypes.c_uint64,
                                ctypes.c_int32, _c_vp, _c_vp, _c_vp]),
-    "nz_state_init": (ctypes.c_int, [_c_vp, ctypes.POINTER(NzSpinState), _c_vp]),
-    "nz_walk": (ctypes.c_int, [_c_vp, ctypes.POINTER(NzSpinState), ctypes.c_uint64,
-                               ctypes.c_int64, ctypes.c_int32, ctypes.c_double, _c_vp, _c_vp,
-                               _c_vp, ctypes.c_uint32, _c_vp]),
+    "nz_walker_create": (ctypes.c_int, [_c_vp, ctypes.c_int64, ctypes.c_uint32,
+                                        ctypes.POINTER(_c_vp)]),
+    "nz_walker_destroy": (ctypes.c_int, [_c_vp]),
+    "nz_walker_set_sort_interval": (ctypes.c_int, [_c_vp, ctypes.c_int32]),
+    "nz_walker_get_step": (ctypes.c_int, [_c_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "nz_walker_set_step": (ctypes.c_int, [_c_vp, ctypes.c_int64]),
+    "nz_walker_set_positions": (ctypes.c_int, [_c_vp, _c_vp, _c_vp]),
+    "nz_walker_seed": (ctypes.c_int, [_c_vp, _c_vp, ctypes.c_int32, ctypes.c_uint64,
+                                      ctypes.c_int32, _c_vp, _c_vp]),
+    "nz_walk": (ctypes.c_int, [_c_vp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double, _c_vp,
+                               _c_vp, _c_vp, ctypes.c_uint32, _c_vp]),
+    "nz_walker_read": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "nz_walker_read_f64": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp]),
+    "nz_walker_counters": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp]),
+    "nz_displacement_moments": (ctypes.c_int, [_c_vp, _c_vp, _c_vp]),
     "nz_project_phase": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_vp, _c_vp, _c_vp,
                                         _c_vp, _c_vp]),
     "nz_signal_reduce": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_vp, _c_vp, _c_vp,
                                         _c_vp, _c_vp]),
-    "nz_displacement_moments": (ctypes.c_int, [_c_vp, ctypes.POINTER(NzSpinState), _c_vp,
-                                               _c_vp]),
 }
 
 EXPORTED = tuple(_SIGS)

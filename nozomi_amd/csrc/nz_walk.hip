@@ -1,14 +1,53 @@
-// The walker kernels: one spin per lane, many steps per launch (the per-spin
-// logic is in nz_walk_core.h).
+// The walker handle and its kernels: resident spin state, spatial sort, the
+// persistent walk kernel (one spin per lane, many steps per launch; the
+// per-spin logic is in nz_walk_core.h) and the read-back kernels.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "nz_walk_core.h"
 
 namespace nz {
 
+// Per-slot resident state.  Slots are ordered by cell (see sort_spins);
+// spin_id maps a slot to its spin, which keys the Philox stream and orders
+// the read-back.
+struct SlotState {
+  int32_t* spin_id;  // [n]
+  int32_t* ucell;    // [3,n] unwrapped cell index
+  double* rel;       // [3,n] position relative to the cell's lower corner
+  int32_t* ucell0;   // [3,n] start cell
+  double* rel0;      // [3,n] start position relative to that cell
+  int32_t* sid;      // [n]
+  float* logsig;     // [n]
+  float* mom;        // [3,n]
+  int32_t* nsub;     // [n] or null
+  int32_t* nguard;   // [n] or null
+};
+
+struct Walker {
+  const Substrate* sub;
+  int64_t n;
+  uint32_t flags;
+  SlotState buf[2];
+  int cur = 0;
+  int32_t* keys = nullptr;    // [n] sort keys
+  int32_t* counts = nullptr;  // [ncells + 1]
+  int32_t* cursor = nullptr;  // [ncells]
+  float* pos_tmp = nullptr;   // [3,n] seeding output
+  int64_t ncells = 0;
+  int32_t sort_interval = 32;
+  int32_t since_sort = 1 << 30;
+  int64_t step = 0;
+  std::vector<void*> allocations;
+  int device = 0;
+};
+
 struct WalkArgs {
   Grid g;
-  nz_spin_state st;
+  SlotState st;
+  int64_t n;
   uint64_t seed;
   int64_t step0;
   int32_t nsteps;
@@ -27,22 +66,18 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
   const Grid& g = A.g;
-  const nz_spin_state& st = A.st;
-  const int64_t n = st.n;
+  const SlotState& st = A.st;
+  const int64_t n = A.n;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < n;
   const int64_t ii = active ? i : 0;
 
   Lane s;
-  int u0[3];
-  double p0[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     s.u[a] = active ? st.ucell[a * n + ii] : 0;
     s.p[a] = active ? st.rel[a * n + ii] : 0.0;
-    u0[a] = active ? st.ucell0[a * n + ii] : 0;
-    p0[a] = active ? st.rel0[a * n + ii] : 0.0;
-    s.m[a] = (active && A.gph && st.mom) ? st.mom[a * n + ii] : 0.0f;
+    s.m[a] = (active && A.gph) ? st.mom[a * n + ii] : 0.0f;
   }
   lane_load_cells(g, s);
   s.logsig = active ? st.logsig[ii] : 0.0f;
@@ -50,7 +85,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
   s.nsub = 0;
   s.nguard = 0;
   const bool do_t2 = (A.flags & NZ_WALK_T2) != 0;
-  const uint32_t spin = (uint32_t)ii;
+  const uint32_t spin = active ? (uint32_t)st.spin_id[ii] : 0u;
 
   for (int t = 0; t < A.nsteps; ++t) {
     const float G = A.gph ? A.gph[t] : 0.0f;
@@ -60,7 +95,9 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
       float v[6];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float d = active ? (float)lane_disp(g, s, u0, p0, a) : 0.0f;
+        const int u0 = active ? st.ucell0[a * n + ii] : 0;
+        const double p0 = active ? st.rel0[a * n + ii] : 0.0;
+        const float d = active ? (float)((double)(s.u[a] - u0) * g.hd[a] + (s.p[a] - p0)) : 0.0f;
         v[a] = d * d;
         v[3 + a] = v[a] * v[a];
       }
@@ -73,61 +110,162 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
   }
 
   if (!active) return;
-  float pos[3], pos0[3];
-  lane_outputs(g, s, u0, p0, pos, pos0);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     st.ucell[a * n + i] = s.u[a];
     st.rel[a * n + i] = s.p[a];
-    st.pos[a * n + i] = pos[a];
-    st.pos0[a * n + i] = pos0[a];
+    if (A.gph) st.mom[a * n + i] = s.m[a];
   }
   st.logsig[i] = s.logsig;
-  st.sig[i] = expf(s.logsig);
   st.sid[i] = (int32_t)s.sid;
-  if (A.gph && st.mom) {
-    st.mom[i] = s.m[0];
-    st.mom[n + i] = s.m[1];
-    st.mom[2 * n + i] = s.m[2];
-  }
   if (st.nsub) st.nsub[i] += s.nsub;
   if (st.nguard) st.nguard[i] += s.nguard;
 }
 
-// Absolute positions -> resident state.
-__global__ void __launch_bounds__(256) state_init_kernel(const Grid g, const nz_spin_state st) {
-  const int64_t n = st.n;
+// Absolute start positions (spin order) -> resident state (slot = spin).
+__global__ void __launch_bounds__(256) init_kernel(const Grid g, const SlotState st, int64_t n,
+                                                   const float* pos) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Lane s;
-  const float pos[3] = {st.pos[i], st.pos[n + i], st.pos[2 * n + i]};
-  lane_init(g, pos, s);
+  const float p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+  lane_init(g, p, s);
+  st.spin_id[i] = (int32_t)i;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     st.ucell[a * n + i] = s.u[a];
     st.ucell0[a * n + i] = s.u[a];
     st.rel[a * n + i] = s.p[a];
     st.rel0[a * n + i] = s.p[a];
-    st.pos0[a * n + i] = pos[a];
-    if (st.mom) st.mom[a * n + i] = 0.0f;
+    st.mom[a * n + i] = 0.0f;
   }
   st.sid[i] = (int32_t)s.sid;
   st.logsig[i] = 0.0f;
-  st.sig[i] = 1.0f;
   if (st.nsub) st.nsub[i] = 0;
   if (st.nguard) st.nguard[i] = 0;
 }
 
-__global__ void __launch_bounds__(256) disp_moments_kernel(const Grid g, const nz_spin_state st,
-                                                           double* out) {
-  const int64_t n = st.n;
+// ---- spatial counting sort by wrapped cell -------------------------------
+__global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ int32_t wrapped_cell(const Grid& g, const int32_t* ucell, int64_t n,
+                                                int64_t i) {
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int w = ucell[a * n + i] % g.ncell[a];
+    c[a] = w < 0 ? w + g.ncell[a] : w;
+  }
+  return (c[2] * g.ncell[1] + c[1]) * g.ncell[0] + c[0];
+}
+
+__global__ void __launch_bounds__(256) sort_keys_kernel(const Grid g, const SlotState st, int64_t n,
+                                                        int32_t* keys, int32_t* counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t k = wrapped_cell(g, st.ucell, n, i);
+  keys[i] = k;
+  atomicAdd(&counts[k], 1);
+}
+
+// Exclusive scan of counts[0..m) in place (counts[m] = total), one workgroup.
+__global__ void __launch_bounds__(1024) scan_kernel(int32_t* counts, int64_t m) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (m + 1023) / 1024;
+  const int64_t b = t * per, e = min<int64_t>(m, b + per);
+  int32_t s = 0;
+  for (int64_t k = b; k < e; ++k) s += counts[k];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+    const int32_t v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int32_t run = (t == 0) ? 0 : part[t - 1];
+  for (int64_t k = b; k < e; ++k) {
+    const int32_t c = counts[k];
+    counts[k] = run;
+    run += c;
+  }
+  if (t == 1023) counts[m] = part[1023];
+}
+
+__global__ void __launch_bounds__(256) sort_scatter_kernel(const SlotState src, const SlotState dst,
+                                                           int64_t n, const int32_t* keys,
+                                                           const int32_t* offsets, int32_t* cursor) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t k = keys[i];
+  const int64_t j = offsets[k] + atomicAdd(&cursor[k], 1);
+  dst.spin_id[j] = src.spin_id[i];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    dst.ucell[a * n + j] = src.ucell[a * n + i];
+    dst.rel[a * n + j] = src.rel[a * n + i];
+    dst.ucell0[a * n + j] = src.ucell0[a * n + i];
+    dst.rel0[a * n + j] = src.rel0[a * n + i];
+    dst.mom[a * n + j] = src.mom[a * n + i];
+  }
+  dst.sid[j] = src.sid[i];
+  dst.logsig[j] = src.logsig[i];
+  if (src.nsub) dst.nsub[j] = src.nsub[i];
+  if (src.nguard) dst.nguard[j] = src.nguard[i];
+}
+
+// ---- read-back in spin order ---------------------------------------------
+__global__ void __launch_bounds__(256) read_kernel(const Grid g, const SlotState st, int64_t n,
+                                                   float* pos, float* pos0, float* sig, float* mom,
+                                                   double* pos_d, double* disp_d, int32_t* nsub,
+                                                   int32_t* nguard) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t k = st.spin_id[i];
+  Lane s;
+  int u0[3];
+  double p0[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    s.u[a] = st.ucell[a * n + i];
+    s.p[a] = st.rel[a * n + i];
+    u0[a] = st.ucell0[a * n + i];
+    p0[a] = st.rel0[a * n + i];
+  }
+  lane_load_cells(g, s);
+  if (pos || pos0) {
+    float x[3], x0[3];
+    lane_outputs(g, s, u0, p0, x, x0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (pos) pos[a * n + k] = x[a];
+      if (pos0) pos0[a * n + k] = x0[a];
+    }
+  }
+  if (sig) sig[k] = expf(st.logsig[i]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (mom) mom[a * n + k] = st.mom[a * n + i];
+    if (pos_d) pos_d[a * n + k] = cell_lo(g, a, s.l[a]) + s.p[a];
+    if (disp_d) disp_d[a * n + k] = lane_disp(g, s, u0, p0, a);
+  }
+  if (nsub) nsub[k] = st.nsub ? st.nsub[i] : 0;
+  if (nguard) nguard[k] = st.nguard ? st.nguard[i] : 0;
+}
+
+__global__ void __launch_bounds__(256) disp_moments_kernel(const Grid g, const SlotState st,
+                                                           int64_t n, double* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float v[6] = {0, 0, 0, 0, 0, 0};
   if (i < n) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const float d = (float)((double)(st.ucell[a * n + i] - st.ucell0[a * n + i]) * g.hd[a] +
-                              (st.rel[a * n + i] - st.rel0[a * n + i]));  // lane_disp
+                              (st.rel[a * n + i] - st.rel0[a * n + i]));
       v[a] = d * d;
       v[3 + a] = v[a] * v[a];
     }
@@ -139,17 +277,46 @@ __global__ void __launch_bounds__(256) disp_moments_kernel(const Grid g, const n
   }
 }
 
-__global__ void zero6(double* p) {
-  if (threadIdx.x < 6) p[threadIdx.x] = 0.0;
+__global__ void zero_f64_kernel(double* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
 }
 
-static int check_state(const nz_spin_state* st, const char* who) {
-  if (!st) return fail(-1, std::string(who) + ": state is NULL");
-  if (st->n <= 0 || st->n > (int64_t)INT32_MAX)
-    return fail(-1, std::string(who) + ": spin count must be in [1, 2^31)");
-  if (!st->pos || !st->pos0 || !st->sig || !st->ucell || !st->rel || !st->ucell0 || !st->rel0 ||
-      !st->logsig || !st->sid)
-    return fail(-1, std::string(who) + ": a required state buffer is NULL");
+static unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+static int walker_alloc(Walker* w, size_t bytes, void** out) {
+  void* p = nullptr;
+  NZ_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+  w->allocations.push_back(p);
+  *out = p;
+  return 0;
+}
+
+static void walker_release(Walker* w) {
+  for (void* p : w->allocations) (void)hipFree(p);
+  w->allocations.clear();
+}
+
+static int sort_spins(Walker* w, hipStream_t stream) {
+  const Grid& g = w->sub->g;
+  const int64_t n = w->n;
+  hipLaunchKernelGGL(fill_i32, dim3(blocks_for(w->ncells + 1)), dim3(256), 0, stream, w->counts,
+                     w->ncells + 1, 0);
+  hipLaunchKernelGGL(fill_i32, dim3(blocks_for(w->ncells)), dim3(256), 0, stream, w->cursor,
+                     w->ncells, 0);
+  hipLaunchKernelGGL(sort_keys_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, g, w->buf[w->cur],
+                     n, w->keys, w->counts);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, w->counts, w->ncells);
+  hipLaunchKernelGGL(sort_scatter_kernel, dim3(blocks_for(n)), dim3(256), 0, stream,
+                     w->buf[w->cur], w->buf[1 - w->cur], n, w->keys, w->counts, w->cursor);
+  NZ_HIP(hipGetLastError());
+  w->cur = 1 - w->cur;
+  w->since_sort = 0;
+  return 0;
+}
+
+static int check_walker(const Walker* w, const char* who) {
+  if (!w) return fail(-1, std::string(who) + ": walker is NULL");
   return 0;
 }
 
@@ -159,55 +326,197 @@ using namespace nz;
 
 extern "C" {
 
-int nz_state_init(nz_substrate_t sub, const nz_spin_state* st, void* stream) {
-  if (!sub) return fail(-1, "nz_state_init: substrate is NULL");
-  if (int rc = check_state(st, "nz_state_init")) return rc;
-  const Substrate* s = (const Substrate*)sub;
-  const int64_t blocks = (st->n + 255) / 256;
-  hipLaunchKernelGGL(state_init_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
-                     s->g, *st);
-  NZ_HIP(hipGetLastError());
+int nz_walker_create(nz_substrate_t sub, int64_t n, uint32_t flags, nz_walker_t* out) {
+  if (!out) return fail(-1, "nz_walker_create: out is NULL");
+  *out = nullptr;
+  if (!sub) return fail(-1, "nz_walker_create: substrate is NULL");
+  if (n <= 0 || n > (int64_t)INT32_MAX) return fail(-1, "nz_walker_create: spin count must be in [1, 2^31)");
+  Walker* w = new Walker();
+  w->sub = (const Substrate*)sub;
+  w->n = n;
+  w->flags = flags;
+  (void)hipGetDevice(&w->device);
+  const Grid& g = w->sub->g;
+  w->ncells = (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
+  int rc = 0;
+  for (int b = 0; b < 2 && !rc; ++b) {
+    SlotState& s = w->buf[b];
+    void* p;
+    if ((rc = walker_alloc(w, sizeof(int32_t) * n, &p))) break;
+    s.spin_id = (int32_t*)p;
+    if ((rc = walker_alloc(w, sizeof(int32_t) * 3 * n, &p))) break;
+    s.ucell = (int32_t*)p;
+    if ((rc = walker_alloc(w, sizeof(double) * 3 * n, &p))) break;
+    s.rel = (double*)p;
+    if ((rc = walker_alloc(w, sizeof(int32_t) * 3 * n, &p))) break;
+    s.ucell0 = (int32_t*)p;
+    if ((rc = walker_alloc(w, sizeof(double) * 3 * n, &p))) break;
+    s.rel0 = (double*)p;
+    if ((rc = walker_alloc(w, sizeof(int32_t) * n, &p))) break;
+    s.sid = (int32_t*)p;
+    if ((rc = walker_alloc(w, sizeof(float) * n, &p))) break;
+    s.logsig = (float*)p;
+    if ((rc = walker_alloc(w, sizeof(float) * 3 * n, &p))) break;
+    s.mom = (float*)p;
+    s.nsub = s.nguard = nullptr;
+    if (flags & NZ_WALKER_COUNTERS) {
+      if ((rc = walker_alloc(w, sizeof(int32_t) * n, &p))) break;
+      s.nsub = (int32_t*)p;
+      if ((rc = walker_alloc(w, sizeof(int32_t) * n, &p))) break;
+      s.nguard = (int32_t*)p;
+    }
+  }
+  void* p = nullptr;
+  if (!rc) rc = walker_alloc(w, sizeof(int32_t) * n, &p);
+  if (!rc) { w->keys = (int32_t*)p; rc = walker_alloc(w, sizeof(int32_t) * (w->ncells + 1), &p); }
+  if (!rc) { w->counts = (int32_t*)p; rc = walker_alloc(w, sizeof(int32_t) * w->ncells, &p); }
+  if (!rc) { w->cursor = (int32_t*)p; rc = walker_alloc(w, sizeof(float) * 3 * n, &p); }
+  if (!rc) w->pos_tmp = (float*)p;
+  if (rc) {
+    walker_release(w);
+    delete w;
+    return rc;
+  }
+  *out = (nz_walker_t)w;
   return 0;
 }
 
-int nz_displacement_moments(nz_substrate_t sub, const nz_spin_state* st, double* out, void* stream) {
-  if (!sub) return fail(-1, "nz_displacement_moments: substrate is NULL");
-  if (int rc = check_state(st, "nz_displacement_moments")) return rc;
-  if (!out) return fail(-1, "nz_displacement_moments: out is NULL");
-  const Substrate* s = (const Substrate*)sub;
-  hipLaunchKernelGGL(zero6, dim3(1), dim3(64), 0, (hipStream_t)stream, out);
-  const int64_t blocks = (st->n + 255) / 256;
-  hipLaunchKernelGGL(disp_moments_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
-                     s->g, *st, out);
-  NZ_HIP(hipGetLastError());
+int nz_walker_destroy(nz_walker_t ww) {
+  if (!ww) return 0;
+  Walker* w = (Walker*)ww;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(w->device);
+  walker_release(w);
+  (void)hipSetDevice(dev);
+  delete w;
   return 0;
 }
 
-int nz_walk(nz_substrate_t sub, const nz_spin_state* st, uint64_t seed, int64_t step0,
-            int32_t nsteps, double dt, const float* g, const float* garea, double* stepmom,
-            uint32_t flags, void* stream) {
-  if (!sub) return fail(-1, "nz_walk: substrate is NULL");
-  if (int rc = check_state(st, "nz_walk")) return rc;
+int nz_walker_set_sort_interval(nz_walker_t ww, int32_t steps) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_set_sort_interval")) return rc;
+  if (steps < 0) return fail(-1, "nz_walker_set_sort_interval: steps < 0");
+  w->sort_interval = steps;
+  return 0;
+}
+
+int nz_walker_get_step(nz_walker_t ww, int64_t* step) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_get_step")) return rc;
+  if (!step) return fail(-1, "nz_walker_get_step: step is NULL");
+  *step = w->step;
+  return 0;
+}
+
+int nz_walker_set_step(nz_walker_t ww, int64_t step) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_set_step")) return rc;
+  if (step < 0 || step > (int64_t)UINT32_MAX) return fail(-1, "nz_walker_set_step: out of range");
+  w->step = step;
+  return 0;
+}
+
+int nz_walker_set_positions(nz_walker_t ww, const float* pos, void* stream) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_set_positions")) return rc;
+  if (!pos) return fail(-1, "nz_walker_set_positions: pos is NULL");
+  w->cur = 0;
+  hipLaunchKernelGGL(init_kernel, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)stream,
+                     w->sub->g, w->buf[0], w->n, pos);
+  NZ_HIP(hipGetLastError());
+  w->step = 0;
+  w->since_sort = 1 << 30;
+  return 0;
+}
+
+int nz_walker_seed(nz_walker_t ww, const int32_t* targets, int32_t ntargets, uint64_t seed,
+                   int32_t max_attempts, int32_t* nfailed, void* stream) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_seed")) return rc;
+  if (int rc = nz_seed((nz_substrate_t)w->sub, w->n, targets, ntargets, seed, max_attempts,
+                       w->pos_tmp, nfailed, stream))
+    return rc;
+  return nz_walker_set_positions(ww, w->pos_tmp, stream);
+}
+
+int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const float* g,
+            const float* garea, double* stepmom, uint32_t flags, void* stream) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walk")) return rc;
   if (nsteps < 0) return fail(-1, "nz_walk: nsteps < 0");
   if (!(dt >= 0.0)) return fail(-1, "nz_walk: dt must be >= 0");
   if ((g == nullptr) != (garea == nullptr)) return fail(-1, "nz_walk: pass both g and garea or neither");
-  if (g && !st->mom) return fail(-1, "nz_walk: phase coefficients given but state.mom is NULL");
-  if (step0 < 0 || step0 + nsteps > (int64_t)UINT32_MAX) return fail(-1, "nz_walk: step index out of range");
-  if (nsteps == 0) return 0;
-  const Substrate* s = (const Substrate*)sub;
-  WalkArgs a;
-  a.g = s->g;
-  a.st = *st;
-  a.seed = seed;
-  a.step0 = step0;
-  a.nsteps = nsteps;
-  a.dt = dt;
-  a.gph = g;
-  a.garea = garea;
-  a.stepmom = stepmom;
-  a.flags = flags;
-  const int64_t blocks = (st->n + 255) / 256;
-  hipLaunchKernelGGL(walk_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, a);
+  if (w->step + nsteps > (int64_t)UINT32_MAX) return fail(-1, "nz_walk: step counter overflow");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t done = 0;
+  while (done < nsteps) {
+    if (w->sort_interval > 0 && w->since_sort >= w->sort_interval)
+      if (int rc = sort_spins(w, s)) return rc;
+    int32_t chunk = nsteps - done;
+    if (w->sort_interval > 0) chunk = std::min(chunk, w->sort_interval - w->since_sort);
+    WalkArgs a;
+    a.g = w->sub->g;
+    a.st = w->buf[w->cur];
+    a.n = w->n;
+    a.seed = seed;
+    a.step0 = w->step;
+    a.nsteps = chunk;
+    a.dt = dt;
+    a.gph = g ? g + done : nullptr;
+    a.garea = garea ? garea + done : nullptr;
+    a.stepmom = stepmom ? stepmom + (int64_t)done * 6 : nullptr;
+    a.flags = flags;
+    hipLaunchKernelGGL(walk_kernel, dim3(blocks_for(w->n)), dim3(256), 0, s, a);
+    NZ_HIP(hipGetLastError());
+    done += chunk;
+    w->step += chunk;
+    w->since_sort += chunk;
+  }
+  return 0;
+}
+
+int nz_walker_read(nz_walker_t ww, float* pos, float* pos0, float* sig, float* mom, void* stream) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_read")) return rc;
+  hipLaunchKernelGGL(read_kernel, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)stream,
+                     w->sub->g, w->buf[w->cur], w->n, pos, pos0, sig, mom, (double*)nullptr,
+                     (double*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
+  NZ_HIP(hipGetLastError());
+  return 0;
+}
+
+int nz_walker_read_f64(nz_walker_t ww, double* pos, double* disp, void* stream) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_read_f64")) return rc;
+  hipLaunchKernelGGL(read_kernel, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)stream,
+                     w->sub->g, w->buf[w->cur], w->n, (float*)nullptr, (float*)nullptr,
+                     (float*)nullptr, (float*)nullptr, pos, disp, (int32_t*)nullptr,
+                     (int32_t*)nullptr);
+  NZ_HIP(hipGetLastError());
+  return 0;
+}
+
+int nz_walker_counters(nz_walker_t ww, int32_t* nsub, int32_t* nguard, void* stream) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_counters")) return rc;
+  if (!(w->flags & NZ_WALKER_COUNTERS))
+    return fail(-1, "nz_walker_counters: walker created without NZ_WALKER_COUNTERS");
+  hipLaunchKernelGGL(read_kernel, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)stream,
+                     w->sub->g, w->buf[w->cur], w->n, (float*)nullptr, (float*)nullptr,
+                     (float*)nullptr, (float*)nullptr, (double*)nullptr, (double*)nullptr, nsub,
+                     nguard);
+  NZ_HIP(hipGetLastError());
+  return 0;
+}
+
+int nz_displacement_moments(nz_walker_t ww, double* out, void* stream) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_displacement_moments")) return rc;
+  if (!out) return fail(-1, "nz_displacement_moments: out is NULL");
+  hipLaunchKernelGGL(zero_f64_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, out, (int64_t)6);
+  hipLaunchKernelGGL(disp_moments_kernel, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)stream,
+                     w->sub->g, w->buf[w->cur], w->n, out);
   NZ_HIP(hipGetLastError());
   return 0;
 }

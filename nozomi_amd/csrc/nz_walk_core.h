@@ -89,38 +89,45 @@ struct ScanAcc {
   int kin;        // sphere index of fin (-1: none above 0)
   int kout;       // sphere index of fout (-1: none)
   bool any_in;    // inside some relevant sphere
-  int nguard;     // candidates re-solved in FP64
+  bool exact;     // a candidate fell in a guard band: redo the scan in FP64
 };
 
-// Lane-invariant inputs of a scan: FP32 copies for the candidate loop and
-// FP64 originals for the re-solves.
+// Lane-invariant inputs of the FP32 candidate loop.
 struct ScanIn {
   float p[3], j[3];
   float a, inv_a, sid;
-  double pd[3], jd[3];
-  double lo[3];  // absolute lower corner of the lane's cell (FP64)
 };
 
-// FP64 re-solve of one candidate (sphere index k): the exact structure and
-// collision tests of the reference for this sphere.
-NZ_HD __noinline__ void visit_exact(const Grid& g, const ScanIn& in, int k, ScanAcc& acc) {
+NZ_HD inline void acc_reset(const Grid& g, ScanAcc& acc) {
+  acc.sid_new = (float)g.nstruct;
+  acc.fin = 0.0f;
+  acc.fout = kLarge;
+  acc.kin = -1;
+  acc.kout = -1;
+  acc.any_in = false;
+  acc.exact = false;
+}
+
+// One candidate in FP64 (sphere index k; p, j relative to the cell corner
+// lo): the exact structure and collision tests of the reference.
+NZ_HD inline void visit_exact(const Grid& g, const double p[3], const double j[3],
+                              const double lo[3], float sid, int k, ScanAcc& acc) {
   const double4 s = g.sph[k];
   const float sidc = (float)g.sph_sid[k];
-  const double dx = in.pd[0] - (s.x - in.lo[0]);
-  const double dy = in.pd[1] - (s.y - in.lo[1]);
-  const double dz = in.pd[2] - (s.z - in.lo[2]);
+  const double dx = p[0] - (s.x - lo[0]);
+  const double dy = p[1] - (s.y - lo[1]);
+  const double dz = p[2] - (s.z - lo[2]);
   const bool inside = (dx * dx + dy * dy + dz * dz - s.w * s.w) < 0;
   if (inside && sidc < acc.sid_new) acc.sid_new = sidc;
-  if (sidc > in.sid) return;
+  if (sidc > sid) return;
   if (inside) {
-    const float f = (float)step_inside_d(dx, dy, dz, s.w, in.jd[0], in.jd[1], in.jd[2]);
+    const float f = (float)step_inside_d(dx, dy, dz, s.w, j[0], j[1], j[2]);
     acc.any_in = true;
     if (acc.fin < f) { acc.fin = f; acc.kin = k; }
-  } else if (sidc < in.sid) {
-    const float f = (float)step_outside_d(dx, dy, dz, s.w, in.jd[0], in.jd[1], in.jd[2]);
+  } else if (sidc < sid) {
+    const float f = (float)step_outside_d(dx, dy, dz, s.w, j[0], j[1], j[2]);
     if (f < acc.fout) { acc.fout = f; acc.kout = k; }
   }
-  acc.nguard++;
 }
 
 // One candidate sphere: the structure test (sim3d_kernel.cu:380-398) and the
@@ -128,8 +135,7 @@ NZ_HD __noinline__ void visit_exact(const Grid& g, const ScanIn& in, int k, Scan
 // lane's cell corner, B = (r^2, (r-tol)^2, (r+tol)^2, sphere index bits).
 // The two root formulas are those of findStepSizeToSphereInsideTol /
 // ...OutsideTol (:77-160) evaluated on one shared quadratic.
-NZ_HD inline void visit(const Grid& g, const ScanIn& in, const float4 A, const float4 B,
-                        ScanAcc& acc) {
+NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, ScanAcc& acc) {
   const float dx = in.p[0] - A.x, dy = in.p[1] - A.y, dz = in.p[2] - A.z;
   const float dd = dx * dx + dy * dy + dz * dz;
   const float din = dd - B.x;
@@ -139,14 +145,10 @@ NZ_HD inline void visit(const Grid& g, const ScanIn& in, const float4 A, const f
   const float c = dd - R2;
   const float disc = b * b - 4.0f * in.a * c;
   const bool relevant = A.w <= in.sid;                         // :418-419
-  // FP32 cannot decide this candidate: re-solve it in FP64
+  // FP32 cannot decide this candidate: the lane's scan is redone in FP64
   if (fabsf(din) < kGuardInside * (dd + B.x) ||
-      (relevant && fabsf(disc) < kGuardDisc * (b * b + 4.0f * in.a * (dd + R2)))) {
-    int k;
-    memcpy(&k, &B.w, sizeof(k));
-    visit_exact(g, in, k, acc);
-    return;
-  }
+      (relevant && fabsf(disc) < kGuardDisc * (b * b + 4.0f * in.a * (dd + R2))))
+    acc.exact = true;
   if (inside && A.w < acc.sid_new) acc.sid_new = A.w;          // :384-396
   const float sq = sqrtf(fmaxf(disc, 0.0f));
   const float q = (b < 0.0f) ? -0.5f * (b - sq) : -0.5f * (b + sq);
@@ -217,14 +219,28 @@ NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, Sca
   for (int k = cs; k < ce; ++k) {
     const float4 A = g.cand_a[k];
     const float4 B = g.cand_b[k];
-    visit(g, in, A, B, acc);
+    visit(in, A, B, acc);
   }
 }
 
-// Every sphere, for the rare sub-step whose segment leaves the grid's lists.
-NZ_HD __noinline__ void scan_all(const Grid& g, const ScanIn& in, ScanAcc& acc) {
-  for (int64_t k = 0; k < g.nspheres; ++k) visit_exact(g, in, (int)k, acc);
-  acc.nguard -= (int)g.nspheres;
+// The scan in FP64: over the cell list [cs, ce), or over every sphere when
+// cs < 0 (a sub-step whose segment leaves the grid's lists).
+NZ_HD inline void scan_exact(const Grid& g, int cs, int ce, const double p[3], const double j[3],
+                             const int l[3], float sid, ScanAcc& acc) {
+  double lo[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) lo[a] = cell_lo(g, a, l[a]);
+  acc_reset(g, acc);
+  if (cs < 0) {
+    for (int64_t k = 0; k < g.nspheres; ++k) visit_exact(g, p, j, lo, sid, (int)k, acc);
+  } else {
+    for (int q = cs; q < ce; ++q) {
+      int k;
+      const float w = g.cand_b[q].w;
+      memcpy(&k, &w, sizeof(k));
+      visit_exact(g, p, j, lo, sid, k, acc);
+    }
+  }
 }
 
 // Resident per-spin state held in registers across the steps of a launch.
@@ -236,7 +252,7 @@ struct Lane {
   float m[3];    // phase moment
   float sid;     // structure id after the last sub-step
   int nsub;      // scatter sub-steps
-  int nguard;    // FP64 candidate re-solves
+  int nguard;    // sub-steps whose scan ran in FP64
 };
 
 NZ_HD inline void lane_load_cells(const Grid& g, Lane& s) {
@@ -275,7 +291,6 @@ NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin
       draw_jump(seed, spin, step, 0u, ds, j);
     }
     ScanAcc acc;
-    ScanIn in;
     for (int pass = 0; pass < 3; ++pass) {
       bool brute = false;
       if (!in_safety(g, s.p, j, jr)) {
@@ -288,27 +303,23 @@ NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin
           brute = true;
         }
       }
+      acc_reset(g, acc);
+      if (!brute) {
+        ScanIn in;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        in.pd[a] = s.p[a];
-        in.jd[a] = j[a];
-        in.p[a] = (float)s.p[a];
-        in.j[a] = (float)j[a];
-        in.lo[a] = cell_lo(g, a, s.l[a]);
+        for (int a = 0; a < 3; ++a) {
+          in.p[a] = (float)s.p[a];
+          in.j[a] = (float)j[a];
+        }
+        in.a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
+        in.inv_a = 1.0f / in.a;
+        in.sid = s.sid;
+        scan_list(g, cs, ce, in, acc);
       }
-      in.a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
-      in.inv_a = 1.0f / in.a;
-      in.sid = s.sid;
-      acc.sid_new = (float)g.nstruct;
-      acc.fin = 0.0f;
-      acc.fout = kLarge;
-      acc.kin = -1;
-      acc.kout = -1;
-      acc.any_in = false;
-      acc.nguard = 0;
-      if (!brute) scan_list(g, cs, ce, in, acc);
-      else scan_all(g, in, acc);
-      s.nguard += acc.nguard;
+      if (brute || acc.exact) {  // rare: the scan in FP64
+        s.nguard++;
+        scan_exact(g, brute ? -1 : cs, ce, s.p, j, s.l, s.sid, acc);
+      }
       if (acc.sid_new == s.sid) break;
       s.sid = acc.sid_new;  // structure changed: redo with the new id
       if (niter == 1) {
@@ -320,13 +331,13 @@ NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin
     double fin = 0.0, fout = (double)kLarge;
     if (acc.kin >= 0) {
       const double4 c = g.sph[acc.kin];
-      fin = step_inside_d(s.p[0] - (c.x - in.lo[0]), s.p[1] - (c.y - in.lo[1]),
-                          s.p[2] - (c.z - in.lo[2]), c.w, j[0], j[1], j[2]);
+      fin = step_inside_d(s.p[0] - (c.x - cell_lo(g, 0, s.l[0])), s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
+                          s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, j[0], j[1], j[2]);
     }
     if (acc.kout >= 0) {
       const double4 c = g.sph[acc.kout];
-      fout = step_outside_d(s.p[0] - (c.x - in.lo[0]), s.p[1] - (c.y - in.lo[1]),
-                            s.p[2] - (c.z - in.lo[2]), c.w, j[0], j[1], j[2]);
+      fout = step_outside_d(s.p[0] - (c.x - cell_lo(g, 0, s.l[0])), s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
+                            s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, j[0], j[1], j[2]);
     }
     jc = MINd(jc, acc.any_in ? fin : (double)kLarge);  // :457
     jc = MINd(jc, fout);                               // :458

@@ -10,7 +10,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import NzSpinState, NzSubstrateInfo, check, lib
+from ._lib import NzSubstrateInfo, check, lib
 
 _c_vp = ctypes.c_void_p
 
@@ -149,81 +149,127 @@ class DeviceSubstrate:
         return pos
 
 
-class SpinState:
-    """Caller-owned resident state of n spins (struct nz_spin_state)."""
+class Walker:
+    """Resident state of n spins on a substrate (nz_walker_*).
 
-    def __init__(self, n, device, with_phase=True, with_nsub=False):
-        f32, f64, i32 = torch.float32, torch.float64, torch.int32
+    The library keeps the state cell-ordered in double precision; read()
+    returns the reference's views in spin order."""
+
+    def __init__(self, sub, n, counters=False, sort_interval=None):
+        self.sub = sub
+        self.device = sub.device
         self.n = int(n)
-        self.device = device
-        self.pos = torch.empty((3, n), dtype=f32, device=device)
-        self.pos0 = torch.empty((3, n), dtype=f32, device=device)
-        self.sig = torch.ones((1, n), dtype=f32, device=device)
-        self.ucell = torch.empty((3, n), dtype=i32, device=device)
-        self.rel = torch.empty((3, n), dtype=f64, device=device)
-        self.ucell0 = torch.empty((3, n), dtype=i32, device=device)
-        self.rel0 = torch.empty((3, n), dtype=f64, device=device)
-        self.logsig = torch.zeros((n,), dtype=f32, device=device)
-        self.sid = torch.empty((n,), dtype=i32, device=device)
-        self.mom = torch.zeros((3, n), dtype=f32, device=device) if with_phase else None
-        self.nsub = torch.zeros((n,), dtype=i32, device=device) if with_nsub else None
-        self.nguard = torch.zeros((n,), dtype=i32, device=device) if with_nsub else None
-        self.step = 0  # global index of the next step (Philox counter word 1)
+        h = _c_vp()
+        with torch.cuda.device(self.device):
+            check(lib().nz_walker_create(sub.handle, self.n,
+                                         _lib.NZ_WALKER_COUNTERS if counters else 0,
+                                         ctypes.byref(h)), "nz_walker_create")
+        self.handle = h
+        self.counters_enabled = bool(counters)
+        if sort_interval is not None:
+            self.set_sort_interval(sort_interval)
 
-    def c_struct(self):
-        return NzSpinState(self.n, _ptr(self.pos), _ptr(self.pos0), _ptr(self.sig),
-                           _ptr(self.ucell), _ptr(self.rel), _ptr(self.ucell0), _ptr(self.rel0),
-                           _ptr(self.logsig), _ptr(self.sid), _ptr(self.mom), _ptr(self.nsub),
-                           _ptr(self.nguard))
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().nz_walker_destroy(self.handle)
+            self.handle = None
 
-    def bytes(self):
-        return sum(t.numel() * t.element_size() for t in (
-            self.pos, self.pos0, self.sig, self.ucell, self.rel, self.ucell0, self.rel0,
-            self.logsig, self.sid, self.mom, self.nsub, self.nguard) if t is not None)
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
+    def set_sort_interval(self, steps):
+        check(lib().nz_walker_set_sort_interval(self.handle, int(steps)), "nz_walker_set_sort_interval")
 
-def state_init(sub, st):
-    with torch.cuda.device(sub.device):
-        cs = st.c_struct()
-        check(lib().nz_state_init(sub.handle, ctypes.byref(cs), _stream_ptr(sub.device)),
-              "nz_state_init")
-    st.step = 0
+    @property
+    def step(self):
+        v = ctypes.c_int64()
+        check(lib().nz_walker_get_step(self.handle, ctypes.byref(v)), "nz_walker_get_step")
+        return v.value
 
+    @step.setter
+    def step(self, value):
+        check(lib().nz_walker_set_step(self.handle, int(value)), "nz_walker_set_step")
 
-def walk(sub, st, seed, nsteps, dt, g=None, garea=None, stepmom=None, t2=True):
-    """Advance st by nsteps (nz_walk).  g/garea: float32 device tensors [nsteps]."""
-    if g is not None:
-        if g.dtype != torch.float32 or garea.dtype != torch.float32:
-            raise ValueError("g and garea must be float32 tensors")
-        if g.numel() < nsteps or garea.numel() < nsteps:
-            raise ValueError("g and garea need nsteps entries")
-    if stepmom is not None and (stepmom.dtype != torch.float64 or stepmom.numel() < 6 * nsteps):
-        raise ValueError("stepmom must be a float64 tensor with 6*nsteps entries")
-    with torch.cuda.device(sub.device):
-        cs = st.c_struct()
-        check(lib().nz_walk(sub.handle, ctypes.byref(cs), int(seed) & (2**64 - 1), st.step,
-                            int(nsteps), float(dt), _ptr(g), _ptr(garea), _ptr(stepmom),
-                            _lib.NZ_WALK_T2 if t2 else 0, _stream_ptr(sub.device)), "nz_walk")
-    st.step += int(nsteps)
+    def set_positions(self, pos):
+        """Start positions [3, n] float32 (tensor on the walker's device)."""
+        if tuple(pos.shape) != (3, self.n) or pos.dtype != torch.float32:
+            raise ValueError(f"positions must be a float32 tensor of shape (3, {self.n})")
+        pos = pos.to(self.device).contiguous()
+        with torch.cuda.device(self.device):
+            check(lib().nz_walker_set_positions(self.handle, _ptr(pos), _stream_ptr(self.device)),
+                  "nz_walker_set_positions")
+        self._keep = pos
 
+    def seed(self, targets=None, seed=0, max_attempts=4096):
+        failed = torch.zeros(1, dtype=torch.int32, device=self.device)
+        t = None if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
+        with torch.cuda.device(self.device):
+            check(lib().nz_walker_seed(self.handle, None if t is None else t.ctypes.data_as(_c_vp),
+                                       0 if t is None else t.size, int(seed) & (2**64 - 1),
+                                       int(max_attempts), _ptr(failed), _stream_ptr(self.device)),
+                  "nz_walker_seed")
+        nf = int(failed.item())
+        if nf:
+            raise RuntimeError(f"{nf} spins could not be seeded in {max_attempts} attempts: "
+                               "the target structures are empty or too small")
 
-def displacement_moments(sub, st, out=None):
-    out = out if out is not None else torch.empty(6, dtype=torch.float64, device=sub.device)
-    with torch.cuda.device(sub.device):
-        cs = st.c_struct()
-        check(lib().nz_displacement_moments(sub.handle, ctypes.byref(cs), _ptr(out),
-                                            _stream_ptr(sub.device)), "nz_displacement_moments")
-    return out
+    def walk(self, seed, nsteps, dt, g=None, garea=None, stepmom=None, t2=True):
+        """Advance nsteps (nz_walk).  g/garea: float32 device tensors [nsteps]."""
+        if g is not None:
+            if g.dtype != torch.float32 or garea.dtype != torch.float32:
+                raise ValueError("g and garea must be float32 tensors")
+            if g.numel() < nsteps or garea.numel() < nsteps:
+                raise ValueError("g and garea need nsteps entries")
+        if stepmom is not None and (stepmom.dtype != torch.float64 or stepmom.numel() < 6 * nsteps):
+            raise ValueError("stepmom must be a float64 tensor with 6*nsteps entries")
+        with torch.cuda.device(self.device):
+            check(lib().nz_walk(self.handle, int(seed) & (2**64 - 1), int(nsteps), float(dt),
+                                _ptr(g), _ptr(garea), _ptr(stepmom),
+                                _lib.NZ_WALK_T2 if t2 else 0, _stream_ptr(self.device)), "nz_walk")
+
+    def read(self, pos=None, pos0=None, sig=None, mom=None):
+        """Fill the given tensors (spin order): pos/pos0/mom [3, n], sig [n]."""
+        with torch.cuda.device(self.device):
+            check(lib().nz_walker_read(self.handle, _ptr(pos), _ptr(pos0), _ptr(sig), _ptr(mom),
+                                       _stream_ptr(self.device)), "nz_walker_read")
+
+    def read_f64(self):
+        """FP64 wrapped positions and unwrapped displacements, numpy [3, n]."""
+        pos = torch.empty((3, self.n), dtype=torch.float64, device=self.device)
+        disp = torch.empty((3, self.n), dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            check(lib().nz_walker_read_f64(self.handle, _ptr(pos), _ptr(disp),
+                                           _stream_ptr(self.device)), "nz_walker_read_f64")
+        return pos.cpu().numpy(), disp.cpu().numpy()
+
+    def counters(self):
+        """Per-spin (scatter sub-steps, FP64 scans), numpy int32 [n] each."""
+        a = torch.empty(self.n, dtype=torch.int32, device=self.device)
+        b = torch.empty(self.n, dtype=torch.int32, device=self.device)
+        with torch.cuda.device(self.device):
+            check(lib().nz_walker_counters(self.handle, _ptr(a), _ptr(b), _stream_ptr(self.device)),
+                  "nz_walker_counters")
+        return a.cpu().numpy(), b.cpu().numpy()
+
+    def displacement_moments(self, out=None):
+        out = out if out is not None else torch.empty(6, dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            check(lib().nz_displacement_moments(self.handle, _ptr(out), _stream_ptr(self.device)),
+                  "nz_displacement_moments")
+        return out
 
 
 def project_phase(mom, dirs, scale=None, out=None):
     """phase[w, i] = scale[w] * dirs[:, w] . mom[:, i]  (nz_project_phase)."""
     n = mom.shape[1]
     W = dirs.shape[1]
+    dirs = dirs.contiguous()
     out = out if out is not None else torch.empty((W, n), dtype=torch.float32, device=mom.device)
-    check(lib().nz_project_phase(n, W, _ptr(mom), _ptr(dirs.contiguous()),
-                                 _ptr(scale), _ptr(out), _stream_ptr(mom.device)),
-          "nz_project_phase")
+    check(lib().nz_project_phase(n, W, _ptr(mom), _ptr(dirs), _ptr(scale), _ptr(out),
+                                 _stream_ptr(mom.device)), "nz_project_phase")
     return out
 
 
@@ -231,20 +277,8 @@ def signal_sums(mom, dirs, scale=None, out=None):
     """FP64 [W, 2] sums of (cos, sin) of the projected phases (nz_signal_reduce)."""
     n = mom.shape[1]
     W = dirs.shape[1]
+    dirs = dirs.contiguous()
     out = out if out is not None else torch.empty((W, 2), dtype=torch.float64, device=mom.device)
-    check(lib().nz_signal_reduce(n, W, _ptr(mom), _ptr(dirs.contiguous()), _ptr(scale),
-                                 _ptr(out), _stream_ptr(mom.device)), "nz_signal_reduce")
+    check(lib().nz_signal_reduce(n, W, _ptr(mom), _ptr(dirs), _ptr(scale), _ptr(out),
+                                 _stream_ptr(mom.device)), "nz_signal_reduce")
     return out
-
-
-def state_f64(sub, st):
-    """FP64 views of the resident state: absolute wrapped positions [3, n] and
-    unwrapped displacements since nz_state_init [3, n] (numpy)."""
-    inf = sub.info()
-    h = torch.tensor(inf["cell"], dtype=torch.float64, device=st.ucell.device)[:, None]
-    n = torch.tensor(inf["ncell"], dtype=torch.int64, device=st.ucell.device)[:, None]
-    L = torch.tensor(sub.L, dtype=torch.float64, device=st.ucell.device)[:, None]
-    u = st.ucell.to(torch.int64)
-    pos = -0.5 * L + torch.remainder(u, n).to(torch.float64) * h + st.rel
-    disp = (u - st.ucell0.to(torch.int64)).to(torch.float64) * h + (st.rel - st.rel0)
-    return pos.cpu().numpy(), disp.cpu().numpy()

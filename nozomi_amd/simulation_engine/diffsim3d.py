@@ -23,12 +23,15 @@ from . import waveforms as wf
 class DiffSim3d:
     """Simulates diffusion within a 3D geometry of structures (diffsim3d.py:8-226)."""
 
-    def __init__(self, geom, nspins, diffdir=None, device=None, cell_size=None, margin=None):
+    def __init__(self, geom, nspins, diffdir=None, device=None, cell_size=None, margin=None,
+                 sort_interval=None, counters=False):
         self.device = rt._require_gpu(device)
         self.cell_size = cell_size
         self.margin = margin
+        self.sort_interval = sort_interval
+        self.counters = counters
         self.sub = None
-        self.state = None
+        self.walker = None
         self.seed = None
         self.set_spins(nspins)
         self.set_geometry(geom)
@@ -105,46 +108,59 @@ class DiffSim3d:
         if initstates_int is None:
             initstates_int = int(np.random.randint(np.iinfo(np.int32).max, dtype=np.int32))
         self.seed = int(initstates_int)
-        self.state = rt.SpinState(self.nspins, self.device, with_phase=True)
+        self.walker = rt.Walker(sub, self.nspins, counters=self.counters,
+                                sort_interval=self.sort_interval)
         if positions is not None:
-            p = torch.as_tensor(np.asarray(positions, dtype=np.float32)
-                                if not torch.is_tensor(positions) else positions)
-            if tuple(p.shape) != (3, self.nspins):
-                raise ValueError(f"positions must be [3, {self.nspins}]")
-            self.state.pos.copy_(p.to(self.device, torch.float32))
+            p = positions if torch.is_tensor(positions) else torch.from_numpy(
+                np.ascontiguousarray(positions, dtype=np.float32))
+            self.walker.set_positions(p.to(self.device, torch.float32))
         else:
             if structures is None:
                 structures = np.arange(self.geom.nstructures + 1, dtype=np.int32)
             # the seeding stream is keyed apart from the walk stream
-            sub.seed(self.nspins, targets=structures, seed=self.seed ^ 0x5EED5EED5EED,
-                     out=self.state.pos)
-        rt.state_init(sub, self.state)
-        self.spins = self.state.pos.cpu().numpy()
+            self.walker.seed(targets=structures, seed=self.seed ^ 0x5EED5EED5EED)
+        f32 = torch.float32
+        self._pos = torch.empty((3, self.nspins), dtype=f32, device=self.device)
+        self._pos0 = torch.empty((3, self.nspins), dtype=f32, device=self.device)
+        self._sig = torch.empty((1, self.nspins), dtype=f32, device=self.device)
+        self._mom = torch.empty((3, self.nspins), dtype=f32, device=self.device)
+        self._fresh = False
         self.issetup = True
+        self.spins = self.spins_d.get()
+
+    def _views(self):
+        if not self._fresh:
+            self.walker.read(self._pos, self._pos0, self._sig, self._mom)
+            self._fresh = True
 
     # -- device views ----------------------------------------------------------
     @property
     def spins_d(self):
-        return DeviceArray(self.state.pos)
+        self._views()
+        return DeviceArray(self._pos)
 
     @property
     def spins0_d(self):
-        return DeviceArray(self.state.pos0)
+        self._views()
+        return DeviceArray(self._pos0)
 
     @property
     def sig_d(self):
-        return DeviceArray(self.state.sig)
+        self._views()
+        return DeviceArray(self._sig)
 
     @property
     def mom_d(self):
         """[3, N] phase moment sum_t G_t x_t (with wrap corrections)."""
-        return DeviceArray(self.state.mom)
+        self._views()
+        return DeviceArray(self._mom)
 
     @property
     def phase_d(self):
         """Phases [ndiffdir * N] of the multi-direction kernel: the projection
         of the moment on every direction (diffsim3d.py:45, sim3d_kernel.cu:880-942)."""
-        ph = rt.project_phase(self.state.mom, self.diffdir)
+        self._views()
+        ph = rt.project_phase(self._mom, self.diffdir)
         return DeviceArray(ph.reshape(-1))
 
     # -- stepping (diffsim3d.py:180-226) ---------------------------------------
@@ -152,22 +168,24 @@ class DiffSim3d:
         if not self.issetup:
             self.setup()
 
+    def _walk(self, nsteps, dt, **kw):
+        self._ensure()
+        self.walker.walk(self.seed, nsteps, dt, **kw)
+        self._fresh = False
+
     def step(self, dt):
         """One time step with the T2 weight (randomWalk3d)."""
-        self._ensure()
-        rt.walk(self.sub, self.state, self.seed, 1, dt, t2=True)
+        self._walk(1, dt, t2=True)
 
     def run(self, nsteps, dt, stepmom=None, t2=True):
-        """nsteps time steps in one launch; stepmom: optional float64 tensor
+        """nsteps time steps in one call; stepmom: optional float64 tensor
         [nsteps, 6] receiving per-step displacement moment sums."""
-        self._ensure()
-        rt.walk(self.sub, self.state, self.seed, int(nsteps), dt, stepmom=stepmom, t2=t2)
+        self._walk(int(nsteps), dt, stepmom=stepmom, t2=t2)
 
     def _phase_step(self, gwave_dt, G, G_area):
-        self._ensure()
         g = torch.tensor([np.float32(G)], dtype=torch.float32, device=self.device)
         ga = torch.tensor([np.float32(G_area)], dtype=torch.float32, device=self.device)
-        rt.walk(self.sub, self.state, self.seed, 1, gwave_dt, g=g, garea=ga, t2=False)
+        self._walk(1, gwave_dt, g=g, garea=ga, t2=False)
 
     def dwi_step(self, gwave_dt, G, G_area_at_each_time_step):
         """One step accumulating x.G per axis (randomWalk3d_phase)."""
@@ -186,7 +204,7 @@ class DiffSim3d:
         sim3d_kernel.cu:162-238).  The arrays are DeviceArray / tensors /
         numpy arrays."""
         self._ensure()
-        s = rt.displacement_moments(self.sub, self.state)
+        s = self.walker.displacement_moments()
         N = float(self.nspins)
         vals = torch.stack([s[0] / (2.0 * current_time * N), s[1] / (2.0 * current_time * N),
                             s[2] / (2.0 * current_time * N), s[0] / N, s[1] / N, s[2] / N,
@@ -202,8 +220,8 @@ class DiffSim3d:
 
 
 class DwiSim3d(DiffSim3d):
-    """DWI phases under a gradient waveform (diffsim3d.py:228-261).  One launch
-    walks every sample of the waveform."""
+    """DWI phases under a gradient waveform (diffsim3d.py:228-261).  The whole
+    waveform is walked in one call."""
 
     def _coefficients(self, gwave):
         g, ga = wf.phase_coefficients(gwave.wave, gwave.dt)
@@ -213,13 +231,12 @@ class DwiSim3d(DiffSim3d):
         if not self.issetup:
             self.setup(structures=structures, initstates_int=initstates)
         g, ga = self._coefficients(gwave)
-        rt.walk(self.sub, self.state, self.seed, len(gwave.wave), gwave.dt, g=g, garea=ga, t2=False)
+        self._walk(len(gwave.wave), gwave.dt, g=g, garea=ga, t2=False)
 
     def simulate(self, gwave, structures=None, initstates=None):
-        """Phases along x, y, z: returns numpy [3, N] (diffsim3d.py:230-244;
-        with the default directions also [ndiffdir, N])."""
+        """Phases along x, y, z: returns numpy [3, N] (diffsim3d.py:230-244)."""
         self._simulate(gwave, structures, initstates)
-        return self.state.mom.cpu().numpy().reshape(3, self.nspins)
+        return self.mom_d.get().reshape(3, self.nspins)
 
     def simulate_multi_directions(self, gwave, structures=None, initstates=None):
         """Phases along every direction of diffdir: numpy [ndiffdir, N]
@@ -231,7 +248,8 @@ class DwiSim3d(DiffSim3d):
         """Complex signal per direction, sum_i exp(i*phase)/N, computed in FP64
         on the device from the current moments; scales multiplies each
         direction's phase (b-shells of one waveform shape)."""
+        self._views()
         sc = None if scales is None else torch.as_tensor(
             np.asarray(scales, dtype=np.float32)).to(self.device)
-        s = rt.signal_sums(self.state.mom, self.diffdir, sc).cpu().numpy()
+        s = rt.signal_sums(self._mom, self.diffdir, sc).cpu().numpy()
         return (s[:, 0] + 1j * s[:, 1]) / self.nspins

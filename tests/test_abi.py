@@ -15,7 +15,7 @@ def _declared():
 
 def test_header_declares_entry_points():
     names = _declared()
-    for n in ("nz_substrate_create", "nz_walk", "nz_seed", "nz_state_init", "nz_signal_reduce"):
+    for n in ("nz_substrate_create", "nz_walk", "nz_seed", "nz_walker_create", "nz_signal_reduce"):
         assert n in names
 
 
@@ -35,5 +35,8 @@ def test_argument_errors_are_reported():
     rc = L.nz_substrate_create(None, 0, 0, None, None, 10.0, 10.0, 10.0, 0.0, 0.0, ctypes.byref(h))
     assert rc < 0
     assert b"structure" in L.nz_last_error_string()
-    rc = L.nz_walk(None, None, 0, 0, 1, 0.01, None, None, None, 0, None)
+    rc = L.nz_walk(None, 0, 1, 0.01, None, None, None, 0, None)
+    assert rc < 0 and b"walker" in L.nz_last_error_string()
+    w = ctypes.c_void_p()
+    rc = L.nz_walker_create(None, 10, 0, ctypes.byref(w))
     assert rc < 0 and b"substrate" in L.nz_last_error_string()
