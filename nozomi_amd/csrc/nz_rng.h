@@ -14,6 +14,7 @@ __device__ __forceinline__ uint32_t nz_umulhi(uint32_t a, uint32_t b) { return _
 __device__ __forceinline__ void nz_sincospif(float x, float* s, float* c) { sincospif(x, s, c); }
 __device__ __forceinline__ float nz_cospif(float x) { return cospif(x); }
 __device__ __forceinline__ float nz_rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float nz_sqrtf(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ void nz_sincospi(double x, double* s, double* c) { sincospi(x, s, c); }
 __device__ __forceinline__ double nz_cospi(double x) { return cospi(x); }
 #else
@@ -26,6 +27,7 @@ inline void nz_sincospif(float x, float* s, float* c) {
 }
 inline float nz_cospif(float x) { return (float)std::cos(M_PI * (double)x); }
 inline float nz_rcpf(float x) { return 1.0f / x; }
+inline float nz_sqrtf(float x) { return std::sqrt(x); }
 inline void nz_sincospi(double x, double* s, double* c) {
   *s = std::sin(M_PI * x);
   *c = std::cos(M_PI * x);

@@ -162,13 +162,40 @@ __device__ __forceinline__ int32_t wrapped_cell(const Grid& g, const int32_t* uc
   return (c[2] * g.ncell[1] + c[1]) * g.ncell[0] + c[0];
 }
 
+// Runs of equal keys within a wavefront (the common case once spins are
+// sorted) are counted with one atomic per run: lane `head` of the run holds
+// its length.
+struct WaveRun {
+  int head;  // lane of the first lane of this lane's run
+  int len;   // run length (meaningful on the head lane)
+};
+
+__device__ __forceinline__ WaveRun wave_run(int32_t key, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const int32_t prev = __shfl_up(key, 1, 64);
+  const bool start = valid && (lane == 0 || key != prev);
+  const unsigned long long heads = __ballot(start);
+  const unsigned long long vmask = __ballot(valid);
+  // head: highest run start at or below this lane
+  const unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+  WaveRun r;
+  r.head = below ? 63 - __clzll(below) : lane;
+  // run end: next run start above the head, or the end of the valid lanes
+  const unsigned long long above = heads & ~((2ull << r.head) - 1 | (r.head == 63 ? ~0ull : 0ull));
+  const int next = above ? __ffsll(above) - 1 : 64;
+  const int last_valid = vmask ? 63 - __clzll(vmask) : -1;
+  r.len = min(next, last_valid + 1) - r.head;
+  return r;
+}
+
 __global__ void __launch_bounds__(256) sort_keys_kernel(const Grid g, const SlotState st, int64_t n,
                                                         int32_t* keys, int32_t* counts) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int32_t k = wrapped_cell(g, st.ucell, n, i);
-  keys[i] = k;
-  atomicAdd(&counts[k], 1);
+  const bool valid = i < n;
+  const int32_t k = valid ? wrapped_cell(g, st.ucell, n, i) : -1;
+  if (valid) keys[i] = k;
+  const WaveRun r = wave_run(k, valid);
+  if (valid && r.head == (int)(threadIdx.x & 63)) atomicAdd(&counts[k], r.len);
 }
 
 // Exclusive scan of counts[0..m) in place (counts[m] = total), one workgroup.
@@ -200,9 +227,15 @@ __global__ void __launch_bounds__(256) sort_scatter_kernel(const SlotState src, 
                                                            int64_t n, const int32_t* keys,
                                                            const int32_t* offsets, int32_t* cursor) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int32_t k = keys[i];
-  const int64_t j = offsets[k] + atomicAdd(&cursor[k], 1);
+  const bool valid = i < n;
+  const int32_t k = valid ? keys[i] : -1;
+  const WaveRun r = wave_run(k, valid);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (valid && r.head == lane) base = atomicAdd(&cursor[k], r.len);
+  base = __shfl(base, r.head, 64);
+  if (!valid) return;
+  const int64_t j = (int64_t)offsets[k] + base + (lane - r.head);
   dst.spin_id[j] = src.spin_id[i];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -226,6 +259,7 @@ __global__ void __launch_bounds__(256) read_kernel(const Grid g, const SlotState
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t k = st.spin_id[i];
+  if (k < 0 || k >= n) return;  // every slot holds a spin id in [0, n)
   Lane s;
   int u0[3];
   double p0[3];

@@ -131,10 +131,11 @@ NZ_HD inline void visit_exact(const Grid& g, const double p[3], const double j[3
 }
 
 // One candidate sphere: the structure test (sim3d_kernel.cu:380-398) and the
-// collision test (:414-455) fused.  A = (cx, cy, cz, sid) relative to the
-// lane's cell corner, B = (r^2, (r-tol)^2, (r+tol)^2, sphere index bits).
-// The two root formulas are those of findStepSizeToSphereInsideTol /
-// ...OutsideTol (:77-160) evaluated on one shared quadratic.
+// collision test (:414-455) fused, without branches.  A = (cx, cy, cz, sid)
+// relative to the lane's cell corner, B = (r^2, (r-tol)^2, (r+tol)^2, sphere
+// index bits).  The two root formulas are those of
+// findStepSizeToSphereInsideTol / ...OutsideTol (:77-160) evaluated on one
+// shared quadratic; the branch rules of both are applied as selects.
 NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, ScanAcc& acc) {
   const float dx = in.p[0] - A.x, dy = in.p[1] - A.y, dz = in.p[2] - A.z;
   const float dd = dx * dx + dy * dy + dz * dz;
@@ -146,32 +147,32 @@ NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, ScanAc
   const float disc = b * b - 4.0f * in.a * c;
   const bool relevant = A.w <= in.sid;                         // :418-419
   // FP32 cannot decide this candidate: the lane's scan is redone in FP64
-  if (fabsf(din) < kGuardInside * (dd + B.x) ||
-      (relevant && fabsf(disc) < kGuardDisc * (b * b + 4.0f * in.a * (dd + R2))))
-    acc.exact = true;
-  if (inside && A.w < acc.sid_new) acc.sid_new = A.w;          // :384-396
-  const float sq = sqrtf(fmaxf(disc, 0.0f));
-  const float q = (b < 0.0f) ? -0.5f * (b - sq) : -0.5f * (b + sq);
+  const bool guard = fabsf(din) < kGuardInside * (dd + B.x) ||
+                     (relevant && fabsf(disc) < kGuardDisc * (b * b + 4.0f * in.a * (dd + R2)));
+  acc.exact = acc.exact || guard;
+  acc.sid_new = (inside && A.w < acc.sid_new) ? A.w : acc.sid_new;  // :384-396
+  const float sq = nz_sqrtf(fmaxf(disc, 0.0f));
+  const float q = -0.5f * (b + ((b < 0.0f) ? -sq : sq));
   const float fpos = q * in.inv_a;
   const float fneg = c * nz_rcpf(q);
-  if (relevant && inside) {                                     // :428-440
-    const float f = (disc < 0.0f) ? 0.0f : MAXr(MAXr(fpos, fneg), 0.0f);
-    acc.any_in = true;
-    if (acc.fin < f) {
-      acc.fin = f;
-      memcpy(&acc.kin, &B.w, sizeof(int));
-    }
-  } else if (relevant && A.w < in.sid) {                        // :441-454
-    float f;
-    if (disc < 0.0f) f = kLarge;
-    else if (fpos > 0.0f && fneg > 0.0f) f = MINr(fpos, fneg);
-    else if ((fpos < 0.0f && fneg < 0.0f) || (b > 0.0f)) f = kLarge;
-    else f = 0.0f;
-    if (f < acc.fout) {
-      acc.fout = f;
-      memcpy(&acc.kout, &B.w, sizeof(int));
-    }
-  }
+  const bool miss = disc < 0.0f;
+  // inside: the furthest non-negative root (:96-115)
+  const float f_in = miss ? 0.0f : MAXr(MAXr(fpos, fneg), 0.0f);
+  // outside: closest positive root, or the tolerance-band rules (:129-159)
+  float f_out = 0.0f;
+  f_out = ((fpos < 0.0f && fneg < 0.0f) || (b > 0.0f)) ? kLarge : f_out;
+  f_out = (fpos > 0.0f && fneg > 0.0f) ? MINr(fpos, fneg) : f_out;
+  f_out = miss ? kLarge : f_out;
+  int idx;
+  memcpy(&idx, &B.w, sizeof(idx));
+  const bool use_in = relevant && inside;                      // :428-440
+  const bool take_in = use_in && (acc.fin < f_in);
+  acc.any_in = acc.any_in || use_in;
+  acc.fin = take_in ? f_in : acc.fin;
+  acc.kin = take_in ? idx : acc.kin;
+  const bool take_out = relevant && !inside && (A.w < in.sid) && (f_out < acc.fout);  // :441-454
+  acc.fout = take_out ? f_out : acc.fout;
+  acc.kout = take_out ? idx : acc.kout;
 }
 
 NZ_HD inline int cell_id(const Grid& g, int lx, int ly, int lz) {
@@ -215,11 +216,19 @@ NZ_HD inline bool rebin(const Grid& g, double p[3], int l[3], int u[3]) {
   return ok;
 }
 
+// The candidate loop, with the next candidate's loads issued before the
+// current one is evaluated.
 NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, ScanAcc& acc) {
+  if (cs >= ce) return;
+  float4 A = g.cand_a[cs];
+  float4 B = g.cand_b[cs];
   for (int k = cs; k < ce; ++k) {
-    const float4 A = g.cand_a[k];
-    const float4 B = g.cand_b[k];
+    const int kn = (k + 1 < ce) ? k + 1 : k;
+    const float4 An = g.cand_a[kn];
+    const float4 Bn = g.cand_b[kn];
     visit(in, A, B, acc);
+    A = An;
+    B = Bn;
   }
 }
 
