@@ -64,7 +64,17 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Steps per launch when per-step moment sums are requested (LDS partials).
+constexpr int kMomChunk = 256;
+
+// Each lane runs its own spin through nsteps time steps.  One iteration of
+// the loop advances every lane by one sub-step; a lane whose step completes
+// starts its next step in the following iteration, so lanes that scatter do
+// not hold up the rest of the wavefront (the reference's per-step multi-
+// collision loop would otherwise run the whole wavefront to its longest
+// lane every step).
 __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
+  __shared__ float smom[kMomChunk * 6];
   const Grid& g = A.g;
   const SlotState& st = A.st;
   const int64_t n = A.n;
@@ -86,27 +96,51 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
   s.nguard = 0;
   const bool do_t2 = (A.flags & NZ_WALK_T2) != 0;
   const uint32_t spin = active ? (uint32_t)st.spin_id[ii] : 0u;
+  int u0[3] = {0, 0, 0};
+  double p0[3] = {0.0, 0.0, 0.0};
+  if (A.stepmom) {
+    for (int k = threadIdx.x; k < A.nsteps * 6; k += blockDim.x) smom[k] = 0.0f;
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      u0[a] = active ? st.ucell0[a * n + ii] : 0;
+      p0[a] = active ? st.rel0[a * n + ii] : 0.0;
+    }
+  }
 
-  for (int t = 0; t < A.nsteps; ++t) {
-    const float G = A.gph ? A.gph[t] : 0.0f;
-    const float Ga = A.gph ? A.garea[t] : 0.0f;
-    walk_step(g, s, A.seed, spin, (uint32_t)(A.step0 + t), A.dt, do_t2, G, Ga, active);
-    if (A.stepmom) {  // moment sums (sim3d_kernel.cu:191-237), summed in FP64
-      float v[6];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const int u0 = active ? st.ucell0[a * n + ii] : 0;
-        const double p0 = active ? st.rel0[a * n + ii] : 0.0;
-        const float d = active ? (float)((double)(s.u[a] - u0) * g.hd[a] + (s.p[a] - p0)) : 0.0f;
-        v[a] = d * d;
-        v[3 + a] = v[a] * v[a];
+  int t = 0;
+  bool started = false;
+  StepState q;
+  q.pending = -1;
+  while (__any(active && t < A.nsteps)) {
+    if (active && t < A.nsteps) {
+      const uint32_t step = (uint32_t)(A.step0 + t);
+      if (!started) {
+        step_begin(g, s, q, A.dt);
+        started = true;
       }
+      if (q.pending >= 0) step_draw(q, A.seed, spin, step);
+      if (substep(g, s, q, A.seed, spin, step, A.dt, do_t2)) {
+        const float G = A.gph ? A.gph[t] : 0.0f;
+        const float Ga = A.gph ? A.garea[t] : 0.0f;
+        step_end(g, s, G, Ga);
+        if (A.stepmom) {  // moment sums (sim3d_kernel.cu:191-237)
 #pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const float sum = wave_sum(v[q]);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&A.stepmom[(int64_t)t * 6 + q], (double)sum);
+          for (int a = 0; a < 3; ++a) {
+            const float d = (float)lane_disp(g, s, u0, p0, a);
+            atomicAdd(&smom[t * 6 + a], d * d);
+            atomicAdd(&smom[t * 6 + 3 + a], d * d * d * d);
+          }
+        }
+        ++t;
+        started = false;
       }
     }
+  }
+  if (A.stepmom) {  // block partials -> FP64 sums
+    __syncthreads();
+    for (int k = threadIdx.x; k < A.nsteps * 6; k += blockDim.x)
+      atomicAdd(&A.stepmom[k], (double)smom[k]);
   }
 
   if (!active) return;
@@ -489,6 +523,7 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
       if (int rc = sort_spins(w, s)) return rc;
     int32_t chunk = nsteps - done;
     if (w->sort_interval > 0) chunk = std::min(chunk, w->sort_interval - w->since_sort);
+    if (stepmom) chunk = std::min(chunk, kMomChunk);
     WalkArgs a;
     a.g = w->sub->g;
     a.st = w->buf[w->cur];

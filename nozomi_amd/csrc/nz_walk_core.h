@@ -232,24 +232,53 @@ NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, Sca
   }
 }
 
-// The scan in FP64: over the cell list [cs, ce), or over every sphere when
-// cs < 0 (a sub-step whose segment leaves the grid's lists).
-NZ_HD inline void scan_exact(const Grid& g, int cs, int ce, const double p[3], const double j[3],
-                             const int l[3], float sid, ScanAcc& acc) {
+// The scan in FP64 (exact re-solve of a sub-step): over the lane's cell list
+// [cs, ce) (mode 0), over the lists of the 27 cells around the lane's cell
+// (mode 1: a segment that leaves the cell's safety box but not its
+// neighbours'), or over every sphere (mode 2).
+enum { kExactList = 0, kExactNeighbours = 1, kExactAll = 2 };
+
+NZ_HD inline void scan_exact(const Grid& g, int mode, int cs, int ce, const double p[3],
+                             const double j[3], const int l[3], float sid, ScanAcc& acc) {
   double lo[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) lo[a] = cell_lo(g, a, l[a]);
   acc_reset(g, acc);
-  if (cs < 0) {
+  if (mode == kExactAll) {
     for (int64_t k = 0; k < g.nspheres; ++k) visit_exact(g, p, j, lo, sid, (int)k, acc);
-  } else {
-    for (int q = cs; q < ce; ++q) {
-      int k;
-      const float w = g.cand_b[q].w;
-      memcpy(&k, &w, sizeof(k));
-      visit_exact(g, p, j, lo, sid, k, acc);
-    }
+    return;
   }
+  const int r = (mode == kExactNeighbours) ? 1 : 0;
+  for (int dz = -r; dz <= r; ++dz)
+    for (int dy = -r; dy <= r; ++dy)
+      for (int dx = -r; dx <= r; ++dx) {
+        int b = cs, e = ce;
+        if (mode == kExactNeighbours) {
+          const int c = cell_id(g, l[0] + dx, l[1] + dy, l[2] + dz);
+          b = g.cell_start[c];
+          e = g.cell_start[c + 1];
+        }
+        for (int q = b; q < e; ++q) {
+          int k;
+          const float w = g.cand_b[q].w;
+          memcpy(&k, &w, sizeof(k));
+          visit_exact(g, p, j, lo, sid, k, acc);
+        }
+      }
+}
+
+// Segment inside the union of the safety boxes of the 27 cells around the
+// lane's cell, all of which exist?
+NZ_HD inline bool in_neighbourhood(const Grid& g, const double p[3], const double j[3], double jr,
+                                   const int l[3]) {
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double e = p[a] + jr * j[a];
+    const double lo = -g.hd[a] - (double)g.margin, hi = 2.0 * g.hd[a] + (double)g.margin;
+    ok = ok && l[a] >= 1 && l[a] <= g.ncell[a] && p[a] >= lo && p[a] <= hi && e >= lo && e <= hi;
+  }
+  return ok;
 }
 
 // Resident per-spin state held in registers across the steps of a launch.
@@ -282,95 +311,120 @@ NZ_HD inline void draw_jump(uint64_t seed, uint32_t spin, uint32_t step, uint32_
   j[2] = ds * n2;
 }
 
-// One time step of one spin: sub-step loop (sim3d_kernel.cu:367-487), phase
-// (:674-676, :880-887), re-base to the cell of the end point and periodic
-// wrap with the phase correction (:489-521, :683-715, :891-942).
-NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin, uint32_t step,
-                            double dt, bool do_t2, float G, float Ga, bool active) {
-  double jr = active ? 1.0 : 0.0;  // :364
-  int niter = 0;
-  double ds = 0.0, j[3] = {0.0, 0.0, 0.0};
-  int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
-  int cs = g.cell_start[cid], ce = g.cell_start[cid + 1];
-  while (jr > (double)kTol && niter < kMaxIter) {  // :373
-    niter++;
-    double jc = jr;
-    if (niter == 1) {  // :400-406
-      ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
-      draw_jump(seed, spin, step, 0u, ds, j);
-    }
-    ScanAcc acc;
-    for (int pass = 0; pass < 3; ++pass) {
-      bool brute = false;
-      if (!in_safety(g, s.p, j, jr)) {
-        if (rebin(g, s.p, s.l, s.u)) {
-          cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
-          cs = g.cell_start[cid];
-          ce = g.cell_start[cid + 1];
-          brute = !in_safety(g, s.p, j, jr);
-        } else {
-          brute = true;
-        }
+// Progress of one spin through its current time step.
+struct StepState {
+  double jr;    // remaining jump fraction (sim3d_kernel.cu:364)
+  double ds;    // jump scale sqrt(2 D dt) of the step
+  double j[3];  // current jump vector
+  int niter;    // sub-steps done in this step
+  int cs, ce;   // candidate list of the lane's cell
+  int pending;  // draw index of the jump still to draw before the next sub-step (-1: none)
+};
+
+// Start a time step.  The first jump (draw 0) is drawn with the structure
+// id of the previous sub-step; substep() redraws it if the structure pass
+// disagrees (:378-406).
+NZ_HD inline void step_begin(const Grid& g, const Lane& s, StepState& q, double dt) {
+  q.jr = 1.0;
+  q.niter = 0;
+  q.pending = 0;
+  q.ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
+  const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
+  q.cs = g.cell_start[cid];
+  q.ce = g.cell_start[cid + 1];
+}
+
+// Draw the pending jump: draw 0 at the start of a step, draw k after the
+// k-th sub-step scattered (:403-405, :476-478).
+NZ_HD inline void step_draw(StepState& q, uint64_t seed, uint32_t spin, uint32_t step) {
+  draw_jump(seed, spin, step, (uint32_t)q.pending, q.ds, q.j);
+  q.pending = -1;
+}
+
+// One sub-step of the multi-collision loop (sim3d_kernel.cu:373-487).
+// Returns true when the step is complete (:373 loop condition).
+NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, uint32_t spin,
+                          uint32_t step, double dt, bool do_t2) {
+  q.niter++;
+  double jc = q.jr;  // :376
+  ScanAcc acc;
+  for (int pass = 0; pass < 3; ++pass) {
+    int exact_mode = -1;
+    if (!in_safety(g, s.p, q.j, q.jr)) {
+      if (rebin(g, s.p, s.l, s.u)) {
+        const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
+        q.cs = g.cell_start[cid];
+        q.ce = g.cell_start[cid + 1];
+        if (!in_safety(g, s.p, q.j, q.jr))
+          exact_mode = in_neighbourhood(g, s.p, q.j, q.jr, s.l) ? kExactNeighbours : kExactAll;
+      } else {
+        exact_mode = kExactAll;
       }
-      acc_reset(g, acc);
-      if (!brute) {
-        ScanIn in;
+    }
+    acc_reset(g, acc);
+    if (exact_mode < 0) {
+      ScanIn in;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          in.p[a] = (float)s.p[a];
-          in.j[a] = (float)j[a];
-        }
-        in.a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
-        in.inv_a = 1.0f / in.a;
-        in.sid = s.sid;
-        scan_list(g, cs, ce, in, acc);
+      for (int a = 0; a < 3; ++a) {
+        in.p[a] = (float)s.p[a];
+        in.j[a] = (float)q.j[a];
       }
-      if (brute || acc.exact) {  // rare: the scan in FP64
-        s.nguard++;
-        scan_exact(g, brute ? -1 : cs, ce, s.p, j, s.l, s.sid, acc);
-      }
-      if (acc.sid_new == s.sid) break;
-      s.sid = acc.sid_new;  // structure changed: redo with the new id
-      if (niter == 1) {
-        ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
-        draw_jump(seed, spin, step, 0u, ds, j);
-      }
+      in.a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
+      in.inv_a = 1.0f / in.a;
+      in.sid = s.sid;
+      scan_list(g, q.cs, q.ce, in, acc);
+      if (acc.exact) exact_mode = kExactList;
     }
-    // the decisive roots in FP64 (the oracle's arithmetic)
-    double fin = 0.0, fout = (double)kLarge;
-    if (acc.kin >= 0) {
-      const double4 c = g.sph[acc.kin];
-      fin = step_inside_d(s.p[0] - (c.x - cell_lo(g, 0, s.l[0])), s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
-                          s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, j[0], j[1], j[2]);
+    if (exact_mode >= 0) {  // rare: the scan in FP64
+      s.nguard++;
+      scan_exact(g, exact_mode, q.cs, q.ce, s.p, q.j, s.l, s.sid, acc);
     }
-    if (acc.kout >= 0) {
-      const double4 c = g.sph[acc.kout];
-      fout = step_outside_d(s.p[0] - (c.x - cell_lo(g, 0, s.l[0])), s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
-                            s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, j[0], j[1], j[2]);
-    }
-    jc = MINd(jc, acc.any_in ? fin : (double)kLarge);  // :457
-    jc = MINd(jc, fout);                               // :458
-    s.p[0] += jc * j[0];                               // :460-462
-    s.p[1] += jc * j[1];
-    s.p[2] += jc * j[2];
-    const float inv_t2 = (float)(1.0 / g.T2[(int)s.sid]);
-    if (jc < jr) {  // scatter (:464-479)
-      jr = sqrt(MAXd(0.0, jr * jr - jc * jc));
-      if (do_t2) s.logsig = fmaf((float)(-jc * jc * dt), inv_t2, s.logsig);
-      s.nsub++;
-      if (jr > (double)kTol && niter < kMaxIter)  // the draw is only used by a further sub-step
-        draw_jump(seed, spin, step, (uint32_t)niter, ds, j);
-    } else {  // coherent (:480-486)
-      jr -= jc;
-      if (do_t2) s.logsig = fmaf((float)(-jc * dt), inv_t2, s.logsig);
+    if (acc.sid_new == s.sid) break;
+    s.sid = acc.sid_new;  // structure changed: redo with the new id
+    if (q.niter == 1) {
+      q.ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
+      draw_jump(seed, spin, step, 0u, q.ds, q.j);
     }
   }
+  // the decisive roots in FP64 (the oracle's arithmetic)
+  double fin = 0.0, fout = (double)kLarge;
+  if (acc.kin >= 0) {
+    const double4 c = g.sph[acc.kin];
+    fin = step_inside_d(s.p[0] - (c.x - cell_lo(g, 0, s.l[0])), s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
+                        s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, q.j[0], q.j[1], q.j[2]);
+  }
+  if (acc.kout >= 0) {
+    const double4 c = g.sph[acc.kout];
+    fout = step_outside_d(s.p[0] - (c.x - cell_lo(g, 0, s.l[0])), s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
+                          s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, q.j[0], q.j[1], q.j[2]);
+  }
+  jc = MINd(jc, acc.any_in ? fin : (double)kLarge);  // :457
+  jc = MINd(jc, fout);                               // :458
+  s.p[0] += jc * q.j[0];                             // :460-462
+  s.p[1] += jc * q.j[1];
+  s.p[2] += jc * q.j[2];
+  const float inv_t2 = (float)(1.0 / g.T2[(int)s.sid]);
+  if (jc < q.jr) {  // scatter (:464-479)
+    q.jr = sqrt(MAXd(0.0, q.jr * q.jr - jc * jc));
+    if (do_t2) s.logsig = fmaf((float)(-jc * jc * dt), inv_t2, s.logsig);
+    s.nsub++;
+    if (q.jr > (double)kTol && q.niter < kMaxIter)  // the draw is only used by a further sub-step
+      q.pending = q.niter;
+  } else {  // coherent (:480-486)
+    q.jr -= jc;
+    if (do_t2) s.logsig = fmaf((float)(-jc * dt), inv_t2, s.logsig);
+  }
+  return !(q.jr > (double)kTol && q.niter < kMaxIter);
+}
+
+// End of a time step: phase after the move and before the wrap (:674-676,
+// :880-887), re-base to the cell holding the end point, periodic wrap with the
+// phase correction (:489-521, :683-715, :891-942).
+NZ_HD inline void step_end(const Grid& g, Lane& s, float G, float Ga) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    // phase after the move, before the wrap
     const double x = cell_lo(g, a, s.l[a]) + s.p[a];
     s.m[a] = fmaf(G, (float)x, s.m[a]);
-    // re-base to the cell holding the end point
     int k = (int)floor(s.p[a] / g.hd[a]);
     double r = s.p[a] - (double)k * g.hd[a];
     if (r < 0.0) { r += g.hd[a]; --k; }
@@ -390,6 +444,19 @@ NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin
     }
     s.l[a] = nl;
   }
+}
+
+// One whole time step of one spin (the sequential form of the above).
+NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin, uint32_t step,
+                            double dt, bool do_t2, float G, float Ga) {
+  StepState q;
+  step_begin(g, s, q, dt);
+  bool done = false;
+  while (!done) {
+    if (q.pending >= 0) step_draw(q, seed, spin, step);
+    done = substep(g, s, q, seed, spin, step, dt, do_t2);
+  }
+  step_end(g, s, G, Ga);
 }
 
 // Displacement since the start of the spin (unwrapped), per axis.

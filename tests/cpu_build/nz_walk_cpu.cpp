@@ -78,7 +78,7 @@ void nzcpu_walk(void* h, uint64_t seed, int64_t step0, int nsteps, double dt, co
     Lane& s = w->lanes[i];
     for (int t = 0; t < nsteps; ++t) {
       const float G = g ? g[t] : 0.0f, Ga = g ? garea[t] : 0.0f;
-      walk_step(w->hg.g, s, seed, (uint32_t)i, (uint32_t)(step0 + t), dt, do_t2 != 0, G, Ga, true);
+      walk_step(w->hg.g, s, seed, (uint32_t)i, (uint32_t)(step0 + t), dt, do_t2 != 0, G, Ga);
     }
     float pp[3], pp0[3];
     lane_outputs(w->hg.g, s, &w->u0[3 * i], &w->p0[3 * i], pp, pp0);

@@ -202,3 +202,26 @@ def test_free_diffusion_empty_arena_step_api():
         assert (sim.spins_d[a, :].get() <= La / 2).all()
     msd = np.sum(((sim.spins_d[0, :] - sim.spins0_d[0, :]) ** 2).get() / spins)
     assert np.isclose(np.abs(msd / 2 / dt / nt), D, rtol=1e-1)
+
+
+def test_step_moments_match_oracle(c0_geometry, oracle_lib):
+    """Per-step displacement moment sums (the narrow-pulse ADC/kurtosis path,
+    sim3d_kernel.cu:162-238) against the oracle's, 300 steps."""
+    g = c0_geometry
+    N, T, dt = 8192, 300, 0.02
+    sim = DiffSim3d(g, N)
+    sim.setup(initstates_int=99)
+    p0 = sim.spins_d.get().astype(np.float64)
+    sm = torch.zeros((T, 6), dtype=torch.float64, device=sim.device)
+    sim.run(T, dt, stepmom=sm)
+    orc = _oracle_for(g, oracle_lib)
+    pos, pos0 = p0.copy(), p0.copy()
+    res = orc.walk(pos, pos0, np.ones(N), seed=99, step0=0, nsteps=T, dt=dt, stepmom=True)
+    ref = res["stepmom"]
+    got = sm.cpu().numpy()
+    rel = np.abs(got / ref - 1)
+    print(f"step moments: max rel diff {rel.max():.2e}")
+    assert rel.max() < 1e-2
+    # and the one-shot reduction equals the last step's sums
+    last = sim.walker.displacement_moments().cpu().numpy()
+    assert np.allclose(last, got[-1], rtol=1e-5)
