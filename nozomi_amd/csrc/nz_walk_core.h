@@ -164,7 +164,7 @@ NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, ScanAc
   f_out = (fpos > 0.0f && fneg > 0.0f) ? MINr(fpos, fneg) : f_out;
   f_out = miss ? kLarge : f_out;
   int idx;
-  memcpy(&idx, &B.w, sizeof(idx));
+  __builtin_memcpy(&idx, &B.w, sizeof(idx));
   const bool use_in = relevant && inside;                      // :428-440
   const bool take_in = use_in && (acc.fin < f_in);
   acc.any_in = acc.any_in || use_in;
@@ -238,15 +238,34 @@ NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, Sca
 // neighbours'), or over every sphere (mode 2).
 enum { kExactList = 0, kExactNeighbours = 1, kExactAll = 2 };
 
-NZ_HD inline void scan_exact(const Grid& g, int mode, int cs, int ce, const double p[3],
-                             const double j[3], const int l[3], float sid, ScanAcc& acc) {
-  double lo[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) lo[a] = cell_lo(g, a, l[a]);
+// The geometry fields the exact scan reads, passed by value so that the rare
+// out-of-line call needs no copy of the whole grid.
+struct ExactGeom {
+  const double4* sph;
+  const int32_t* sph_sid;
+  const int32_t* cell_start;
+  const float4* cand_b;
+  int64_t nspheres;
+  int32_t nloc0, nloc1;
+  int32_t nstruct;
+};
+
+NZ_HD __noinline__ ScanAcc scan_exact_call(ExactGeom eg, int mode, int cs, int ce, double px,
+                                           double py, double pz, double jx, double jy, double jz,
+                                           double lox, double loy, double loz, int lx, int ly,
+                                           int lz, float sid) {
+  const double p[3] = {px, py, pz};
+  const double j[3] = {jx, jy, jz};
+  const double lo[3] = {lox, loy, loz};
+  Grid g;  // only the fields visit_exact and acc_reset read
+  g.sph = eg.sph;
+  g.sph_sid = eg.sph_sid;
+  g.nstruct = eg.nstruct;
+  ScanAcc acc;
   acc_reset(g, acc);
   if (mode == kExactAll) {
-    for (int64_t k = 0; k < g.nspheres; ++k) visit_exact(g, p, j, lo, sid, (int)k, acc);
-    return;
+    for (int64_t k = 0; k < eg.nspheres; ++k) visit_exact(g, p, j, lo, sid, (int)k, acc);
+    return acc;
   }
   const int r = (mode == kExactNeighbours) ? 1 : 0;
   for (int dz = -r; dz <= r; ++dz)
@@ -254,17 +273,33 @@ NZ_HD inline void scan_exact(const Grid& g, int mode, int cs, int ce, const doub
       for (int dx = -r; dx <= r; ++dx) {
         int b = cs, e = ce;
         if (mode == kExactNeighbours) {
-          const int c = cell_id(g, l[0] + dx, l[1] + dy, l[2] + dz);
-          b = g.cell_start[c];
-          e = g.cell_start[c + 1];
+          const int c = ((lz + dz) * eg.nloc1 + (ly + dy)) * eg.nloc0 + (lx + dx);
+          b = eg.cell_start[c];
+          e = eg.cell_start[c + 1];
         }
         for (int q = b; q < e; ++q) {
           int k;
-          const float w = g.cand_b[q].w;
-          memcpy(&k, &w, sizeof(k));
+          const float w = eg.cand_b[q].w;
+          __builtin_memcpy(&k, &w, sizeof(k));
           visit_exact(g, p, j, lo, sid, k, acc);
         }
       }
+  return acc;
+}
+
+NZ_HD inline void scan_exact(const Grid& g, int mode, int cs, int ce, const double p[3],
+                             const double j[3], const int l[3], float sid, ScanAcc& acc) {
+  ExactGeom eg;
+  eg.sph = g.sph;
+  eg.sph_sid = g.sph_sid;
+  eg.cell_start = g.cell_start;
+  eg.cand_b = g.cand_b;
+  eg.nspheres = g.nspheres;
+  eg.nloc0 = g.nloc[0];
+  eg.nloc1 = g.nloc[1];
+  eg.nstruct = g.nstruct;
+  acc = scan_exact_call(eg, mode, cs, ce, p[0], p[1], p[2], j[0], j[1], j[2], cell_lo(g, 0, l[0]),
+                        cell_lo(g, 1, l[1]), cell_lo(g, 2, l[2]), l[0], l[1], l[2], sid);
 }
 
 // Segment inside the union of the safety boxes of the 27 cells around the
@@ -484,7 +519,7 @@ NZ_HD inline void lane_init(const Grid& g, const float pos[3], Lane& s) {
   for (int k = g.cell_start[cid]; k < g.cell_start[cid + 1]; ++k) {
     int idx;
     const float4 B = g.cand_b[k];
-    memcpy(&idx, &B.w, sizeof(idx));
+    __builtin_memcpy(&idx, &B.w, sizeof(idx));
     const double4 c = g.sph[idx];
     const double dx = s.p[0] - (c.x - cell_lo(g, 0, s.l[0]));
     const double dy = s.p[1] - (c.y - cell_lo(g, 1, s.l[1]));
