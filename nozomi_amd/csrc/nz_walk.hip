@@ -24,6 +24,7 @@ struct SlotState {
   float* mom;        // [3,n]
   int32_t* nsub;     // [n] or null
   int32_t* nguard;   // [n] or null
+  uint32_t* tstep;   // [n] global index of the spin's next step
 };
 
 struct Walker {
@@ -33,11 +34,11 @@ struct Walker {
   SlotState buf[2];
   int cur = 0;
   int32_t* keys = nullptr;    // [n] sort keys
-  int32_t* counts = nullptr;  // [ncells + 1]
-  int32_t* cursor = nullptr;  // [ncells]
+  int32_t* counts = nullptr;  // [nkeys + 1]
+  int32_t* cursor = nullptr;  // [nkeys]
   float* pos_tmp = nullptr;   // [3,n] seeding output
-  int64_t ncells = 0;
-  int32_t sort_interval = 32;
+  int64_t nkeys = 0;          // sort key range: 2 x cells (intra / extra)
+  int32_t sort_interval = 64;
   int32_t since_sort = 1 << 30;
   int64_t step = 0;
   std::vector<void*> allocations;
@@ -49,8 +50,10 @@ struct WalkArgs {
   SlotState st;
   int64_t n;
   uint64_t seed;
-  int64_t step0;
-  int32_t nsteps;
+  int64_t base;      // global step of g[0] / garea[0] (the nz_walk call's first step)
+  int64_t target;    // the wavefront runs until all its spins reached this step
+  int64_t limit;     // a spin may run ahead up to this step
+  int64_t mom_base;  // global step of stepmom[0] for this launch
   double dt;
   const float* gph;
   const float* garea;
@@ -67,13 +70,24 @@ __device__ __forceinline__ float wave_sum(float v) {
 // Steps per launch when per-step moment sums are requested (LDS partials).
 constexpr int kMomChunk = 256;
 
-// Each lane runs its own spin through nsteps time steps.  One iteration of
-// the loop advances every lane by one sub-step; a lane whose step completes
-// starts its next step in the following iteration, so lanes that scatter do
-// not hold up the rest of the wavefront (the reference's per-step multi-
-// collision loop would otherwise run the whole wavefront to its longest
-// lane every step).
-__global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
+// Each lane runs its own spin.  One iteration of the loop advances every
+// lane by one sub-step; a lane whose step completes starts its next step in
+// the following iteration, so lanes that scatter do not hold up the rest of
+// the wavefront.  The wavefront runs until all its spins reached `target`;
+// meanwhile spins that are ahead keep stepping up to `limit` (the end of the
+// walk), so collision-poor spins do not idle at every launch boundary.
+// Register budget of the walk kernel: waves per SIMD the compiler must fit
+// (0 = unconstrained).
+#ifndef NZ_WALK_WAVES_PER_EU
+#define NZ_WALK_WAVES_PER_EU 0
+#endif
+#if NZ_WALK_WAVES_PER_EU > 0
+#define NZ_WALK_ATTR __attribute__((amdgpu_waves_per_eu(NZ_WALK_WAVES_PER_EU)))
+#else
+#define NZ_WALK_ATTR
+#endif
+
+__global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A) {
   __shared__ float smom[kMomChunk * 6];
   const Grid& g = A.g;
   const SlotState& st = A.st;
@@ -96,10 +110,11 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
   s.nguard = 0;
   const bool do_t2 = (A.flags & NZ_WALK_T2) != 0;
   const uint32_t spin = active ? (uint32_t)st.spin_id[ii] : 0u;
+  int64_t t = active ? (int64_t)st.tstep[ii] : A.target;
   int u0[3] = {0, 0, 0};
   double p0[3] = {0.0, 0.0, 0.0};
   if (A.stepmom) {
-    for (int k = threadIdx.x; k < A.nsteps * 6; k += blockDim.x) smom[k] = 0.0f;
+    for (int k = threadIdx.x; k < (int)(A.target - A.mom_base) * 6; k += blockDim.x) smom[k] = 0.0f;
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -108,28 +123,28 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
     }
   }
 
-  int t = 0;
   bool started = false;
   StepState q;
   q.pending = -1;
-  while (__any(active && t < A.nsteps)) {
-    if (active && t < A.nsteps) {
-      const uint32_t step = (uint32_t)(A.step0 + t);
+  while (__any(t < A.target)) {
+    if (active && t < A.limit) {
+      const uint32_t step = (uint32_t)t;
       if (!started) {
         step_begin(g, s, q, A.dt);
         started = true;
       }
       if (q.pending >= 0) step_draw(q, A.seed, spin, step);
       if (substep(g, s, q, A.seed, spin, step, A.dt, do_t2)) {
-        const float G = A.gph ? A.gph[t] : 0.0f;
-        const float Ga = A.gph ? A.garea[t] : 0.0f;
+        const float G = A.gph ? A.gph[t - A.base] : 0.0f;
+        const float Ga = A.gph ? A.garea[t - A.base] : 0.0f;
         step_end(g, s, G, Ga);
         if (A.stepmom) {  // moment sums (sim3d_kernel.cu:191-237)
+          const int k = (int)(t - A.mom_base) * 6;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
             const float d = (float)lane_disp(g, s, u0, p0, a);
-            atomicAdd(&smom[t * 6 + a], d * d);
-            atomicAdd(&smom[t * 6 + 3 + a], d * d * d * d);
+            atomicAdd(&smom[k + a], d * d);
+            atomicAdd(&smom[k + 3 + a], d * d * d * d);
           }
         }
         ++t;
@@ -139,8 +154,8 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
   }
   if (A.stepmom) {  // block partials -> FP64 sums
     __syncthreads();
-    for (int k = threadIdx.x; k < A.nsteps * 6; k += blockDim.x)
-      atomicAdd(&A.stepmom[k], (double)smom[k]);
+    for (int k = threadIdx.x; k < (int)(A.target - A.mom_base) * 6; k += blockDim.x)
+      atomicAdd(&A.stepmom[(A.mom_base - A.base) * 6 + k], (double)smom[k]);
   }
 
   if (!active) return;
@@ -152,6 +167,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkArgs A) {
   }
   st.logsig[i] = s.logsig;
   st.sid[i] = (int32_t)s.sid;
+  st.tstep[i] = (uint32_t)t;
   if (st.nsub) st.nsub[i] += s.nsub;
   if (st.nguard) st.nguard[i] += s.nguard;
 }
@@ -175,8 +191,14 @@ __global__ void __launch_bounds__(256) init_kernel(const Grid g, const SlotState
   }
   st.sid[i] = (int32_t)s.sid;
   st.logsig[i] = 0.0f;
+  st.tstep[i] = 0u;
   if (st.nsub) st.nsub[i] = 0;
   if (st.nguard) st.nguard[i] = 0;
+}
+
+__global__ void fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // ---- spatial counting sort by wrapped cell -------------------------------
@@ -226,7 +248,12 @@ __global__ void __launch_bounds__(256) sort_keys_kernel(const Grid g, const Slot
                                                         int32_t* keys, int32_t* counts) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
-  const int32_t k = valid ? wrapped_cell(g, st.ucell, n, i) : -1;
+  // intra-structure spins first, then extra-structure spins; cell order
+  // within each class (spins with similar collision rates share wavefronts)
+  const int64_t ncells = (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
+  const int32_t k = valid ? wrapped_cell(g, st.ucell, n, i) +
+                                (st.sid[i] >= g.nstruct ? (int32_t)ncells : 0)
+                          : -1;
   if (valid) keys[i] = k;
   const WaveRun r = wave_run(k, valid);
   if (valid && r.head == (int)(threadIdx.x & 63)) atomicAdd(&counts[k], r.len);
@@ -281,6 +308,7 @@ __global__ void __launch_bounds__(256) sort_scatter_kernel(const SlotState src, 
   }
   dst.sid[j] = src.sid[i];
   dst.logsig[j] = src.logsig[i];
+  dst.tstep[j] = src.tstep[i];
   if (src.nsub) dst.nsub[j] = src.nsub[i];
   if (src.nguard) dst.nguard[j] = src.nguard[i];
 }
@@ -368,13 +396,13 @@ static void walker_release(Walker* w) {
 static int sort_spins(Walker* w, hipStream_t stream) {
   const Grid& g = w->sub->g;
   const int64_t n = w->n;
-  hipLaunchKernelGGL(fill_i32, dim3(blocks_for(w->ncells + 1)), dim3(256), 0, stream, w->counts,
-                     w->ncells + 1, 0);
-  hipLaunchKernelGGL(fill_i32, dim3(blocks_for(w->ncells)), dim3(256), 0, stream, w->cursor,
-                     w->ncells, 0);
+  hipLaunchKernelGGL(fill_i32, dim3(blocks_for(w->nkeys + 1)), dim3(256), 0, stream, w->counts,
+                     w->nkeys + 1, 0);
+  hipLaunchKernelGGL(fill_i32, dim3(blocks_for(w->nkeys)), dim3(256), 0, stream, w->cursor,
+                     w->nkeys, 0);
   hipLaunchKernelGGL(sort_keys_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, g, w->buf[w->cur],
                      n, w->keys, w->counts);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, w->counts, w->ncells);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, w->counts, w->nkeys);
   hipLaunchKernelGGL(sort_scatter_kernel, dim3(blocks_for(n)), dim3(256), 0, stream,
                      w->buf[w->cur], w->buf[1 - w->cur], n, w->keys, w->counts, w->cursor);
   NZ_HIP(hipGetLastError());
@@ -405,7 +433,7 @@ int nz_walker_create(nz_substrate_t sub, int64_t n, uint32_t flags, nz_walker_t*
   w->flags = flags;
   (void)hipGetDevice(&w->device);
   const Grid& g = w->sub->g;
-  w->ncells = (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
+  w->nkeys = 2 * (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
   int rc = 0;
   for (int b = 0; b < 2 && !rc; ++b) {
     SlotState& s = w->buf[b];
@@ -426,6 +454,8 @@ int nz_walker_create(nz_substrate_t sub, int64_t n, uint32_t flags, nz_walker_t*
     s.logsig = (float*)p;
     if ((rc = walker_alloc(w, sizeof(float) * 3 * n, &p))) break;
     s.mom = (float*)p;
+    if ((rc = walker_alloc(w, sizeof(uint32_t) * n, &p))) break;
+    s.tstep = (uint32_t*)p;
     s.nsub = s.nguard = nullptr;
     if (flags & NZ_WALKER_COUNTERS) {
       if ((rc = walker_alloc(w, sizeof(int32_t) * n, &p))) break;
@@ -436,8 +466,8 @@ int nz_walker_create(nz_substrate_t sub, int64_t n, uint32_t flags, nz_walker_t*
   }
   void* p = nullptr;
   if (!rc) rc = walker_alloc(w, sizeof(int32_t) * n, &p);
-  if (!rc) { w->keys = (int32_t*)p; rc = walker_alloc(w, sizeof(int32_t) * (w->ncells + 1), &p); }
-  if (!rc) { w->counts = (int32_t*)p; rc = walker_alloc(w, sizeof(int32_t) * w->ncells, &p); }
+  if (!rc) { w->keys = (int32_t*)p; rc = walker_alloc(w, sizeof(int32_t) * (w->nkeys + 1), &p); }
+  if (!rc) { w->counts = (int32_t*)p; rc = walker_alloc(w, sizeof(int32_t) * w->nkeys, &p); }
   if (!rc) { w->cursor = (int32_t*)p; rc = walker_alloc(w, sizeof(float) * 3 * n, &p); }
   if (!rc) w->pos_tmp = (float*)p;
   if (rc) {
@@ -482,6 +512,10 @@ int nz_walker_set_step(nz_walker_t ww, int64_t step) {
   if (int rc = check_walker(w, "nz_walker_set_step")) return rc;
   if (step < 0 || step > (int64_t)UINT32_MAX) return fail(-1, "nz_walker_set_step: out of range");
   w->step = step;
+  hipLaunchKernelGGL(fill_u32, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)0,
+                     w->buf[w->cur].tstep, w->n, (uint32_t)step);
+  NZ_HIP(hipGetLastError());
+  NZ_HIP(hipStreamSynchronize((hipStream_t)0));
   return 0;
 }
 
@@ -517,31 +551,34 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
   if ((g == nullptr) != (garea == nullptr)) return fail(-1, "nz_walk: pass both g and garea or neither");
   if (w->step + nsteps > (int64_t)UINT32_MAX) return fail(-1, "nz_walk: step counter overflow");
   hipStream_t s = (hipStream_t)stream;
-  int32_t done = 0;
-  while (done < nsteps) {
+  const int64_t base = w->step, end = w->step + nsteps;
+  int64_t target = base;
+  while (target < end) {
     if (w->sort_interval > 0 && w->since_sort >= w->sort_interval)
       if (int rc = sort_spins(w, s)) return rc;
-    int32_t chunk = nsteps - done;
-    if (w->sort_interval > 0) chunk = std::min(chunk, w->sort_interval - w->since_sort);
-    if (stepmom) chunk = std::min(chunk, kMomChunk);
+    int64_t chunk = end - target;
+    if (w->sort_interval > 0) chunk = std::min<int64_t>(chunk, w->sort_interval - w->since_sort);
+    if (stepmom) chunk = std::min<int64_t>(chunk, kMomChunk);
     WalkArgs a;
     a.g = w->sub->g;
     a.st = w->buf[w->cur];
     a.n = w->n;
     a.seed = seed;
-    a.step0 = w->step;
-    a.nsteps = chunk;
+    a.base = base;
+    a.mom_base = target;
+    a.target = target + chunk;
+    a.limit = stepmom ? a.target : end;  // per-step sums need the wavefront in step
     a.dt = dt;
-    a.gph = g ? g + done : nullptr;
-    a.garea = garea ? garea + done : nullptr;
-    a.stepmom = stepmom ? stepmom + (int64_t)done * 6 : nullptr;
+    a.gph = g;
+    a.garea = garea;
+    a.stepmom = stepmom;
     a.flags = flags;
     hipLaunchKernelGGL(walk_kernel, dim3(blocks_for(w->n)), dim3(256), 0, s, a);
     NZ_HIP(hipGetLastError());
-    done += chunk;
-    w->step += chunk;
-    w->since_sort += chunk;
+    target += chunk;
+    w->since_sort += (int32_t)chunk;
   }
+  w->step = end;
   return 0;
 }
 

@@ -82,6 +82,31 @@ NZ_HD inline double step_outside_d(double dx, double dy, double dz, double r, do
   return 0;
 }
 
+// Both reference roots in one routine: findStepSizeToSphereInsideTol when
+// `inside`, ...OutsideTol otherwise (sim3d_kernel.cu:77-160), same FP64
+// arithmetic as step_inside_d / step_outside_d.
+NZ_HD inline double root_d(bool inside, double dx, double dy, double dz, double r, double jx,
+                           double jy, double jz) {
+  const double tol = (double)kTol;
+  const double d2r2 = dx * dx + dy * dy + dz * dz - r * r;
+  if (inside && d2r2 > 0) return (double)kLarge;
+  const double R = inside ? r - tol : r + tol;
+  const double a = jx * jx + jy * jy + jz * jz;
+  const double b = 2 * (jx * dx + jy * dy + jz * dz);
+  const double c = dx * dx + dy * dy + dz * dz - R * R;
+  double sq = b * b - 4 * a * c;
+  if (sq < 0) return inside ? 0.0 : (double)kLarge;
+  sq = sqrt(sq);
+  double q = -0.5 * (b + sq);
+  if (b < 0) q = -0.5 * (b - sq);
+  const double fpos = q / a, fneg = c / q;
+  if (inside) return MAXd(MAXd(fpos, fneg), 0);
+  if (d2r2 < 0) return MAXd(fpos, fneg);
+  if (fpos > 0 && fneg > 0) return MINd(fpos, fneg);
+  if ((fpos < 0 && fneg < 0) || (b > 0)) return (double)kLarge;
+  return 0;
+}
+
 struct ScanAcc {
   float sid_new;  // smallest id of a containing sphere (box id if none)
   float fin;      // furthest exit fraction among containing spheres
@@ -238,8 +263,14 @@ NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, Sca
 // neighbours'), or over every sphere (mode 2).
 enum { kExactList = 0, kExactNeighbours = 1, kExactAll = 2 };
 
-// The geometry fields the exact scan reads, passed by value so that the rare
-// out-of-line call needs no copy of the whole grid.
+// The exact scan is inlined by default; building with
+// -DNZ_EXACT_ATTR=__noinline__ moves it out of line (a call costs the kernel
+// the callee's registers, so inlining is usually the better trade).
+#ifndef NZ_EXACT_ATTR
+#define NZ_EXACT_ATTR inline
+#endif
+
+// The geometry fields the exact scan reads, passed by value.
 struct ExactGeom {
   const double4* sph;
   const int32_t* sph_sid;
@@ -250,7 +281,7 @@ struct ExactGeom {
   int32_t nstruct;
 };
 
-NZ_HD __noinline__ ScanAcc scan_exact_call(ExactGeom eg, int mode, int cs, int ce, double px,
+NZ_HD NZ_EXACT_ATTR ScanAcc scan_exact_call(ExactGeom eg, int mode, int cs, int ce, double px,
                                            double py, double pz, double jx, double jy, double jz,
                                            double lox, double loy, double loz, int lx, int ly,
                                            int lz, float sid) {
@@ -421,17 +452,30 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
       draw_jump(seed, spin, step, 0u, q.ds, q.j);
     }
   }
-  // the decisive roots in FP64 (the oracle's arithmetic)
-  double fin = 0.0, fout = (double)kLarge;
-  if (acc.kin >= 0) {
-    const double4 c = g.sph[acc.kin];
-    fin = step_inside_d(s.p[0] - (c.x - cell_lo(g, 0, s.l[0])), s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
-                        s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, q.j[0], q.j[1], q.j[2]);
-  }
-  if (acc.kout >= 0) {
-    const double4 c = g.sph[acc.kout];
-    fout = step_outside_d(s.p[0] - (c.x - cell_lo(g, 0, s.l[0])), s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
-                          s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, q.j[0], q.j[1], q.j[2]);
+  // the decisive roots in FP64 (the oracle's arithmetic).  A root whose FP32
+  // value exceeds the smaller of jr and the other root by more than FP32
+  // round-off cannot decide jc = min(jr, fin, fout); it is not re-solved.
+  double fin = acc.any_in ? (double)acc.fin : (double)kLarge;
+  double fout = (double)acc.fout;
+  {
+    const float fi = acc.any_in ? acc.fin : kLarge;
+    const float lim_in = fminf(acc.fout, (float)q.jr) * 1.0001f + 1e-6f;
+    const float lim_out = fminf(fi, (float)q.jr) * 1.0001f + 1e-6f;
+    const bool need_in = acc.kin >= 0 && acc.any_in && fi <= lim_in;
+    const bool need_out = acc.kout >= 0 && acc.fout <= lim_out;
+    if (acc.any_in && acc.kin < 0) fin = 0.0;
+#pragma unroll 1
+    for (int r = 0; r < 2; ++r) {
+      const bool want = (r == 0) ? need_in : need_out;
+      if (want) {
+        const double4 c = g.sph[(r == 0) ? acc.kin : acc.kout];
+        const double v = root_d(r == 0, s.p[0] - (c.x - cell_lo(g, 0, s.l[0])),
+                                s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
+                                s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, q.j[0], q.j[1], q.j[2]);
+        if (r == 0) fin = v;
+        else fout = v;
+      }
+    }
   }
   jc = MINd(jc, acc.any_in ? fin : (double)kLarge);  // :457
   jc = MINd(jc, fout);                               // :458
