@@ -126,8 +126,11 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A
   bool started = false;
   StepState q;
   q.pending = -1;
-  while (__any(t < A.target)) {
-    if (active && t < A.limit) {
+  // A spin only stops at a step boundary: the sub-step state of a step in
+  // progress is not kept across launches.
+  while (__any(t < A.target || started)) {
+    const bool behind = __any(t < A.target);
+    if (active && (started || (t < A.limit && (t < A.target || behind)))) {
       const uint32_t step = (uint32_t)t;
       if (!started) {
         step_begin(g, s, q, A.dt);
