@@ -40,19 +40,39 @@ def parse():
     ap.add_argument("--box", type=float, default=60.0)
     ap.add_argument("--cell", type=float, default=0.0)
     ap.add_argument("--margin", type=float, default=0.0)
-    ap.add_argument("--sort", type=int, default=32, help="steps between spatial re-sorts")
-    ap.add_argument("--cpu-sample", type=int, default=20000,
+    ap.add_argument("--sort", type=int, default=64, help="steps between spatial re-sorts")
+    ap.add_argument("--cpu-sample", type=int, default=200000,
                     help="spins of the CPU-baseline subsample (0 = skip)")
-    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=5000)
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only copy a buffer of this many MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain device copy of known size: the FETCH_SIZE / WRITE_SIZE counters
+    of this kernel give the counters' scale (bytes read and written are each
+    mib MiB per dispatch)."""
+    import torch
+    n = int(mib * 2**20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 4 * n, "dispatches": 5}), flush=True)
 
 
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only (the walker is "
+                                   "single-GPU; spins would shard by range); run with --gpus 1"}))
         sys.exit(2)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     import torch
 
     from nozomi_amd import runtime as rt
@@ -68,6 +88,7 @@ def main():
     sub = geom.device_substrate(cell_size=args.cell, margin=args.margin, device=dev)
     info = sub.info()
     walker = rt.Walker(sub, N, sort_interval=args.sort)
+    walker.time_kernels(True)
     start = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=args.seed + 1)
     g = torch.from_numpy(prot.g[:T].copy()).to(dev)
     ga = torch.from_numpy(prot.garea[:T].copy()).to(dev)
@@ -78,6 +99,7 @@ def main():
     stream = torch.cuda.current_stream(dev)
 
     walk_ms = []
+    launch_ms = []
 
     def one_pass(record):
         walker.set_positions(start)
@@ -88,8 +110,10 @@ def main():
         e1.record(stream)
         walker.read(mom=mom)
         rt.signal_sums(mom, dirs, scales, out=sums)
+        ms, launches = walker.kernel_time()
         if record:
             walk_ms.append((e0, e1))
+            launch_ms.append(ms / launches)
 
     for _ in range(args.warmup):
         one_pass(False)
@@ -105,16 +129,20 @@ def main():
     spin_steps = float(N) * T * args.steps
     value = spin_steps / elapsed
     kern_ms = float(np.mean(walk_ms))
-    kern_rate = float(N) * T / (kern_ms * 1e-3)
 
     # roofline of the walk kernel (dominant): FP32 vector work per spin-step
     # (1 + s) * C * 56 + 40 with C = mean candidates per cell, s = mean
-    # scatter sub-steps per step (measured below on a subsample)
+    # scatter sub-steps per step (measured below on a subsample); one launch
+    # advances N spins by sort_interval steps
     sbar = measure_sbar(geom, sub, prot, dev, args.seed)
     cbar = info["ncand_mean_box"]
     flop_per = (1.0 + sbar) * cbar * 56.0 + 40.0
-    achieved_tf = kern_rate * flop_per / 1e12
-    hbm_gbs = kern_rate * 24.0 / 1e9
+    launch_avg_ms = float(np.mean(launch_ms))
+    steps_per_launch = min(args.sort, T) if args.sort > 0 else T
+    launch_units = float(N) * steps_per_launch
+    achieved_tf = launch_units * flop_per / (launch_avg_ms * 1e-3) / 1e12
+    hbm_gbs = launch_units * 24.0 / (launch_avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic(N, steps_per_launch, info)
 
     out = {
         "metric": "spin-steps/s at 1e6 spins",
@@ -137,12 +165,15 @@ def main():
             "cells": list(info["ncell"]), "cell_um": info["cell"][0], "margin_um": info["margin"],
             "mean_candidates": cbar, "scatter_per_step": sbar, "sort_interval": args.sort,
         },
-        "walk_kernel_ms": kern_ms,
+        "walk_ms_per_pass": kern_ms,
+        "walk_spin_steps_per_s": float(N) * T / (kern_ms * 1e-3),
+        "walk_kernel_launch_ms": launch_avg_ms,
         "signals_per_s": prot.nsignals * args.steps / elapsed,
         "roofline": {
             "bound": "valu", "achieved": achieved_tf, "peak": FP32_VECTOR_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved_tf / FP32_VECTOR_PEAK_TFLOPS, "traffic": None,
-            "flop_per_spin_step": flop_per,
+            "unit": "TFLOP/s", "frac": achieved_tf / FP32_VECTOR_PEAK_TFLOPS,
+            "traffic": traffic, "kernel": "walk_kernel", "flop_per_spin_step": flop_per,
+            "spin_steps_per_launch": launch_units,
         },
         "roofline_hbm": {
             "bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -153,6 +184,23 @@ def main():
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(geom, prot, start, args)
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(n, steps_per_launch, info):
+    """HBM bytes per walk-kernel launch from the committed counter summary
+    (profiles/walk_kernel_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch,
+    scaled by the copy-kernel calibration), when it was measured on this
+    workload shape; else None."""
+    path = os.path.join(ROOT, "profiles", "walk_kernel_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (t.get("spins") != n or t.get("steps_per_launch") != steps_per_launch
+            or t.get("cells") != list(info["ncell"])):
+        return None
+    return t.get("bytes_per_launch")
 
 
 def measure_sbar(geom, sub, prot, dev, seed, n=65536, steps=200):

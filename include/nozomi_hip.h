@@ -61,8 +61,9 @@ const char* nz_last_error_string(void);
  *                  lower wins, sim3d_kernel.cu:53-54)
  *   D_host, T2_host : nstructures + 1 entries, the box last (diffsim3d.py:95-103)
  *   L*       : box edges (um); periodic cube [-L/2, L/2]
- *   cell_size: target cell edge (um, 0 = auto); the grid tiles the box exactly
- *   margin   : per-cell list margin (um, 0 = auto from the jump length)
+ *   cell_size: target cell edge (um, 0 = 1.5); the grid tiles the box exactly
+ *   margin   : per-cell list margin (um, 0 = 0.8); a sub-step that would
+ *              leave the cell's box grown by the margin re-bins the spin
  * Synchronous (builds on the host, copies to the current device). */
 int nz_substrate_create(const double* spheres_host, int64_t nspheres, int32_t nstructures,
                         const double* D_host, const double* T2_host, double Lx, double Ly,
@@ -148,6 +149,14 @@ int nz_walker_counters(nz_walker_t w, int32_t* nsub, int32_t* nguard, void* stre
  * computeDiffusionCoefficientsAndKurtosis (sim3d_kernel.cu:162-238) before the
  * host scaling D = S/(2tN), K = S/N.  out: device double [6], overwritten. */
 int nz_displacement_moments(nz_walker_t w, double* out, void* stream);
+
+/* Launch timing of the walk kernel (no counterpart in the reference, which
+ * has no timing inside its launches): with enable != 0 every walk-kernel
+ * launch of nz_walk is bracketed by HIP events on the launch stream.
+ * nz_walker_kernel_time waits for the recorded launches, returns their total
+ * duration (ms) and count, and starts a new tally. */
+int nz_walker_time_kernels(nz_walker_t w, int32_t enable);
+int nz_walker_kernel_time(nz_walker_t w, double* ms_total, int64_t* launches);
 
 /* Phases of W directions from the moments: phase[w, i] = scale[w] *
  * (dirs[0,w] m_x + dirs[1,w] m_y + dirs[2,w] m_z).  Replaces the per-step

@@ -66,6 +66,9 @@ _SIGS = {
     "nz_walker_read_f64": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp]),
     "nz_walker_counters": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp]),
     "nz_displacement_moments": (ctypes.c_int, [_c_vp, _c_vp, _c_vp]),
+    "nz_walker_time_kernels": (ctypes.c_int, [_c_vp, ctypes.c_int32]),
+    "nz_walker_kernel_time": (ctypes.c_int, [_c_vp, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_int64)]),
     "nz_project_phase": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_vp, _c_vp, _c_vp,
                                         _c_vp, _c_vp]),
     "nz_signal_reduce": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_vp, _c_vp, _c_vp,

@@ -58,8 +58,8 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
       return -1;
     }
   }
-  if (cell_size <= 0) cell_size = 2.0;
-  if (margin <= 0) margin = 1.0;
+  if (cell_size <= 0) cell_size = 1.5;
+  if (margin <= 0) margin = 0.8;
   Grid& g = out.g;
   std::memset(&g, 0, sizeof(g));
   std::memset(&out.info, 0, sizeof(out.info));

@@ -43,6 +43,11 @@ struct Walker {
   int64_t step = 0;
   std::vector<void*> allocations;
   int device = 0;
+  // walk_kernel launch timing (nz_walker_time_kernels): one start/stop event
+  // pair per launch since the last nz_walker_kernel_time
+  bool timing = false;
+  std::vector<hipEvent_t> events;
+  size_t events_used = 0;
 };
 
 struct WalkArgs {
@@ -77,9 +82,11 @@ constexpr int kMomChunk = 256;
 // meanwhile spins that are ahead keep stepping up to `limit` (the end of the
 // walk), so collision-poor spins do not idle at every launch boundary.
 // Register budget of the walk kernel: waves per SIMD the compiler must fit
-// (0 = unconstrained).
+// (0 = unconstrained).  Three waves (168 VGPRs, a few spills) beat the
+// unconstrained two (172 VGPRs) by 1.23x on the 200-axon substrate; four
+// waves (128 VGPRs, more spills) measure the same as three.
 #ifndef NZ_WALK_WAVES_PER_EU
-#define NZ_WALK_WAVES_PER_EU 0
+#define NZ_WALK_WAVES_PER_EU 3
 #endif
 #if NZ_WALK_WAVES_PER_EU > 0
 #define NZ_WALK_ATTR __attribute__((amdgpu_waves_per_eu(NZ_WALK_WAVES_PER_EU)))
@@ -394,6 +401,22 @@ static int walker_alloc(Walker* w, size_t bytes, void** out) {
 static void walker_release(Walker* w) {
   for (void* p : w->allocations) (void)hipFree(p);
   w->allocations.clear();
+  for (hipEvent_t e : w->events) (void)hipEventDestroy(e);
+  w->events.clear();
+  w->events_used = 0;
+}
+
+// The event pair of the next timed launch (created on first use).
+static int next_event_pair(Walker* w, hipEvent_t* start, hipEvent_t* stop) {
+  while (w->events.size() < w->events_used + 2) {
+    hipEvent_t e;
+    NZ_HIP(hipEventCreate(&e));
+    w->events.push_back(e);
+  }
+  *start = w->events[w->events_used];
+  *stop = w->events[w->events_used + 1];
+  w->events_used += 2;
+  return 0;
 }
 
 static int sort_spins(Walker* w, hipStream_t stream) {
@@ -576,8 +599,14 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
     a.garea = garea;
     a.stepmom = stepmom;
     a.flags = flags;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (w->timing) {
+      if (int rc = next_event_pair(w, &e0, &e1)) return rc;
+      NZ_HIP(hipEventRecord(e0, s));
+    }
     hipLaunchKernelGGL(walk_kernel, dim3(blocks_for(w->n)), dim3(256), 0, s, a);
     NZ_HIP(hipGetLastError());
+    if (w->timing) NZ_HIP(hipEventRecord(e1, s));
     target += chunk;
     w->since_sort += (int32_t)chunk;
   }
@@ -616,6 +645,31 @@ int nz_walker_counters(nz_walker_t ww, int32_t* nsub, int32_t* nguard, void* str
                      (float*)nullptr, (float*)nullptr, (double*)nullptr, (double*)nullptr, nsub,
                      nguard);
   NZ_HIP(hipGetLastError());
+  return 0;
+}
+
+int nz_walker_time_kernels(nz_walker_t ww, int32_t enable) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_time_kernels")) return rc;
+  w->timing = enable != 0;
+  w->events_used = 0;
+  return 0;
+}
+
+int nz_walker_kernel_time(nz_walker_t ww, double* ms_total, int64_t* launches) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walker_kernel_time")) return rc;
+  if (!ms_total || !launches) return fail(-1, "nz_walker_kernel_time: NULL output");
+  double total = 0.0;
+  for (size_t k = 0; k + 1 < w->events_used; k += 2) {
+    NZ_HIP(hipEventSynchronize(w->events[k + 1]));
+    float ms = 0.0f;
+    NZ_HIP(hipEventElapsedTime(&ms, w->events[k], w->events[k + 1]));
+    total += ms;
+  }
+  *ms_total = total;
+  *launches = (int64_t)(w->events_used / 2);
+  w->events_used = 0;
   return 0;
 }
 

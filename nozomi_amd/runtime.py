@@ -254,6 +254,20 @@ class Walker:
                   "nz_walker_counters")
         return a.cpu().numpy(), b.cpu().numpy()
 
+    def time_kernels(self, enable=True):
+        """Bracket every walk-kernel launch with HIP events on its stream."""
+        check(lib().nz_walker_time_kernels(self.handle, 1 if enable else 0), "nz_walker_time_kernels")
+
+    def kernel_time(self):
+        """(total ms, launches) of the walk-kernel launches timed since the
+        last call; waits for them."""
+        ms = ctypes.c_double()
+        k = ctypes.c_int64()
+        with torch.cuda.device(self.device):
+            check(lib().nz_walker_kernel_time(self.handle, ctypes.byref(ms), ctypes.byref(k)),
+                  "nz_walker_kernel_time")
+        return ms.value, k.value
+
     def displacement_moments(self, out=None):
         out = out if out is not None else torch.empty(6, dtype=torch.float64, device=self.device)
         with torch.cuda.device(self.device):
