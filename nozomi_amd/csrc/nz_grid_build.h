@@ -23,7 +23,7 @@ namespace nz {
 struct HostGrid {
   Grid g;  // scalar fields set; pointers bound by bind_host() or by the upload
   std::vector<int32_t> start;
-  std::vector<float4> ca, cb;
+  std::vector<float4> cand;  // a, b records interleaved
   std::vector<double4> sph;
   std::vector<int32_t> sid;
   std::vector<double> D, T2;
@@ -31,8 +31,7 @@ struct HostGrid {
 
   void bind_host() {
     g.cell_start = start.data();
-    g.cand_a = ca.data();
-    g.cand_b = cb.data();
+    g.cand = cand.data();
     g.sph = sph.data();
     g.sph_sid = sid.data();
     g.D = D.data();
@@ -106,8 +105,14 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
         err = "candidate lists exceed 2^31 entries; lower cell_size or margin";
         return -1;
       }
-      out.ca.assign((size_t)cnt[ncells], make_float4(0, 0, 0, 0));
-      out.cb.assign((size_t)cnt[ncells], make_float4(0, 0, 0, 0));
+      for (int64_t c = 0; c < ncells; ++c)
+        if (cnt[c + 1] - cnt[c] > kMaxListLen) {
+          err = "a cell lists more than " + std::to_string(kMaxListLen) +
+                " spheres; lower cell_size or margin";
+          return -1;
+        }
+      // one padding record after the last list (the scan loads one ahead)
+      out.cand.assign(2 * (size_t)cnt[ncells] + 2, make_float4(0, 0, 0, 0));
       fill.assign(cnt.begin(), cnt.end() - 1);
     }
     for (int64_t i = 0; i < S; ++i) {
@@ -133,9 +138,10 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
               cnt[c + 1]++;
             } else {
               const int64_t k = fill[c]++;
-              out.ca[k] = make_float4((float)(cx[i] - cell_lo(0, x)), (float)(cy[i] - cell_lo(1, y)),
-                                      (float)(cz[i] - cell_lo(2, z)), (float)sd[i]);
-              out.cb[k] = bv;
+              out.cand[2 * k] = make_float4((float)(cx[i] - cell_lo(0, x)),
+                                            (float)(cy[i] - cell_lo(1, y)),
+                                            (float)(cz[i] - cell_lo(2, z)), (float)sd[i]);
+              out.cand[2 * k + 1] = bv;
             }
           }
     }

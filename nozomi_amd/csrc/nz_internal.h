@@ -15,6 +15,8 @@ namespace nz {
 constexpr float kTol = 1e-4f;
 constexpr float kLarge = 1e6f;
 constexpr int kMaxIter = 100;
+// Longest candidate list of a cell (the FP32 scan keys a list position in 8 bits).
+constexpr int kMaxListLen = 255;
 
 // Device view of a substrate.  The box [-L/2, L/2] per axis is tiled by
 // ncell[a] cells of edge h[a]; one halo cell on each side (local index 0 and
@@ -33,12 +35,13 @@ struct Grid {
   float margin;        // safety margin of the lists
   int32_t nstruct;     // number of structures; the box has id nstruct
   int64_t nspheres;
-  // CSR candidate lists over the nloc^3 local cells, cell-relative coords:
+  // CSR candidate lists over the nloc^3 local cells; entry e is the 32-byte
+  // record cand[2e] = a, cand[2e+1] = b with cell-relative coordinates:
   //   a = (cx - lo_x, cy - lo_y, cz - lo_z, sid)
   //   b = (r^2, (r-tol)^2, (r+tol)^2, sphere index as int bits)
+  // A list holds at most 255 entries.
   const int32_t* cell_start;  // [ncells + 1]
-  const float4* cand_a;
-  const float4* cand_b;
+  const float4* cand;         // [2 * entries + 2]: a padding record at the end
   // every sphere, absolute, FP64: the exact re-solves and the rare sub-step
   // that leaves the grid's lists
   const double4* sph;         // (cx, cy, cz, r)

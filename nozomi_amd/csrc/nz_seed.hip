@@ -63,8 +63,8 @@ __global__ void __launch_bounds__(256) seed_kernel(const SeedArgs A) {
     int min_n = box_target ? INT32_MAX : g.nstruct;
     const int cid = (l[2] * g.nloc[1] + l[1]) * g.nloc[0] + l[0];
     for (int q = g.cell_start[cid]; q < g.cell_start[cid + 1]; ++q) {
-      const float4 S = g.cand_a[q];
-      const float4 B = g.cand_b[q];
+      const float4 S = g.cand[2 * (int64_t)q];
+      const float4 B = g.cand[2 * (int64_t)q + 1];
       const float dx = p[0] - S.x, dy = p[1] - S.y, dz = p[2] - S.z;
       if (dx * dx + dy * dy + dz * dz < B.x) {  // sim3d_kernel.cu:299
         const int sid = (int)S.w;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "nz_walk_core.h"
@@ -39,6 +40,7 @@ struct Walker {
   float* pos_tmp = nullptr;   // [3,n] seeding output
   int64_t nkeys = 0;          // sort key range: 2 x cells (intra / extra)
   int32_t sort_interval = 64;
+  int32_t xcd_chunk = 64;     // NZ_WALK_XCD_CHUNK in the environment overrides it
   int32_t since_sort = 1 << 30;
   int64_t step = 0;
   std::vector<void*> allocations;
@@ -64,7 +66,25 @@ struct WalkArgs {
   const float* garea;
   double* stepmom;
   uint32_t flags;
+  int32_t xcd_chunk;  // slot blocks per XCD chunk (0: no remap)
 };
+
+// Workgroups of a launch are dispatched to the 8 XCDs round-robin (block b
+// runs on XCD b % 8) and every XCD has its own L2.  The remap hands XCD x
+// chunks of `chunk` consecutive (cell-sorted) slot blocks: within each group
+// of 8*chunk blocks, XCD x processes blocks [x*chunk, (x+1)*chunk), so the
+// blocks one L2 serves at a time cover a few compact regions of space
+// instead of the whole span the chip is working on, while every XCD still
+// gets an equal share of each part of the sorted order (the cost per spin
+// differs between regions).  A final partial group keeps the identity map.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb, int64_t chunk) {
+  constexpr int64_t kXcd = 8;
+  if (chunk <= 0) return b;
+  const int64_t group = kXcd * chunk;
+  const int64_t g0 = (b / group) * group;
+  if (g0 + group > nb) return b;
+  return g0 + (b % kXcd) * chunk + (b - g0) / kXcd;
+}
 
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
@@ -99,7 +119,8 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A
   const Grid& g = A.g;
   const SlotState& st = A.st;
   const int64_t n = A.n;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
+  const int64_t i = blk * blockDim.x + threadIdx.x;
   const bool active = i < n;
   const int64_t ii = active ? i : 0;
 
@@ -460,6 +481,7 @@ int nz_walker_create(nz_substrate_t sub, int64_t n, uint32_t flags, nz_walker_t*
   (void)hipGetDevice(&w->device);
   const Grid& g = w->sub->g;
   w->nkeys = 2 * (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
+  if (const char* e = getenv("NZ_WALK_XCD_CHUNK")) w->xcd_chunk = std::max(0, atoi(e));
   int rc = 0;
   for (int b = 0; b < 2 && !rc; ++b) {
     SlotState& s = w->buf[b];
@@ -599,6 +621,7 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
     a.garea = garea;
     a.stepmom = stepmom;
     a.flags = flags;
+    a.xcd_chunk = w->xcd_chunk;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (w->timing) {
       if (int rc = next_event_pair(w, &e0, &e1)) return rc;

@@ -120,8 +120,36 @@ struct ScanAcc {
 // Lane-invariant inputs of the FP32 candidate loop.
 struct ScanIn {
   float p[3], j[3];
-  float a, inv_a, sid;
+  float a4, inv_a, sid;  // 4 |j|^2, 1 / |j|^2, structure id assumed for the scan
 };
+
+// Accumulators of the FP32 candidate loop.  A decision and the list position
+// of the candidate that made it share one 32-bit key: the root's float bits
+// (non-negative, so they order as integers) with the low 8 mantissa bits
+// replaced by the position + 1.  The furthest exit is the largest key, the
+// closest entry the smallest; the roots keep 2^-15 relative precision, which
+// is all the later FP64 re-solve needs to know which roots can decide.
+// Candidates inside a guard band are left out and counted; the one position
+// is kept for an FP64 visit after the loop.
+struct ScanKeys {
+  uint32_t kin;    // 0: no relevant containing sphere
+  uint32_t kout;   // 0xffffffff: no obstacle
+  float sid_new;   // smallest id of a containing sphere (box id if none)
+  int32_t gcnt;    // candidates in a guard band
+  uint32_t gpos;   // position + 1 of the last of them
+};
+constexpr uint32_t kKeyMask = 0x7fffff00u;  // list positions (< kMaxListLen) in the low 8 bits
+
+NZ_HD inline uint32_t f2u(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, sizeof(u));
+  return u;
+}
+NZ_HD inline float u2f(uint32_t u) {
+  float f;
+  __builtin_memcpy(&f, &u, sizeof(f));
+  return f;
+}
 
 NZ_HD inline void acc_reset(const Grid& g, ScanAcc& acc) {
   acc.sid_new = (float)g.nstruct;
@@ -158,46 +186,49 @@ NZ_HD inline void visit_exact(const Grid& g, const double p[3], const double j[3
 // One candidate sphere: the structure test (sim3d_kernel.cu:380-398) and the
 // collision test (:414-455) fused, without branches.  A = (cx, cy, cz, sid)
 // relative to the lane's cell corner, B = (r^2, (r-tol)^2, (r+tol)^2, sphere
-// index bits).  The two root formulas are those of
-// findStepSizeToSphereInsideTol / ...OutsideTol (:77-160) evaluated on one
-// shared quadratic; the branch rules of both are applied as selects.
-NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, ScanAcc& acc) {
+// index bits).  One quadratic a f^2 + b f + c (c = dd - (r -/+ tol)^2) serves
+// both reference roots (findStepSizeToSphereInsideTol / ...OutsideTol,
+// :77-160), whose branch rules reduce to: with q = -(b - sqrt(disc))/2 for
+// b <= 0 (q >= 0, far root q/a, near root c/q) and q = -(b + sqrt(disc))/2
+// for b > 0,
+//   inside : max(0, b <= 0 ? q/a : c/q), 0 on a miss;
+//   outside: b > 0 or a miss -> no hit, else max(0, c/q).
+NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, uint32_t pos1,
+                        ScanKeys& k) {
   const float dx = in.p[0] - A.x, dy = in.p[1] - A.y, dz = in.p[2] - A.z;
-  const float dd = dx * dx + dy * dy + dz * dz;
+  const float dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
   const float din = dd - B.x;
-  const float b = 2.0f * (in.j[0] * dx + in.j[1] * dy + in.j[2] * dz);
+  const float hb = fmaf(in.j[2], dz, fmaf(in.j[1], dy, in.j[0] * dx));
+  const float b = hb + hb;
   const bool inside = din < 0.0f;                              // isInsideSphere < 0
   const float R2 = inside ? B.y : B.z;
   const float c = dd - R2;
-  const float disc = b * b - 4.0f * in.a * c;
+  const float b2 = b * b;
+  const float disc = fmaf(-in.a4, c, b2);
   const bool relevant = A.w <= in.sid;                         // :418-419
-  // FP32 cannot decide this candidate: the lane's scan is redone in FP64
-  const bool guard = fabsf(din) < kGuardInside * (dd + B.x) ||
-                     (relevant && fabsf(disc) < kGuardDisc * (b * b + 4.0f * in.a * (dd + R2)));
-  acc.exact = acc.exact || guard;
-  acc.sid_new = (inside && A.w < acc.sid_new) ? A.w : acc.sid_new;  // :384-396
+  // FP32 cannot decide this candidate: it is re-solved in FP64 after the loop
+  // (bitwise bool algebra keeps the masks in scalar registers)
+  const bool g_in = fabsf(din) < kGuardInside * (dd + B.x);
+  const bool g_disc = fabsf(disc) < kGuardDisc * fmaf(in.a4, dd + R2, b2);
+  const bool guard = g_in | (relevant & g_disc);
+  k.gcnt += guard ? 1 : 0;
+  k.gpos = guard ? pos1 : k.gpos;
+  const bool take_sid = inside & !guard & (A.w < k.sid_new);  // :384-396
+  k.sid_new = take_sid ? A.w : k.sid_new;
   const float sq = nz_sqrtf(fmaxf(disc, 0.0f));
-  const float q = -0.5f * (b + ((b < 0.0f) ? -sq : sq));
-  const float fpos = q * in.inv_a;
-  const float fneg = c * nz_rcpf(q);
+  const bool away = b > 0.0f;
+  const float q = -0.5f * (b + (away ? sq : -sq));
+  const float near = c * nz_rcpf(q);
+  const float root = (inside & !away) ? q * in.inv_a : near;
   const bool miss = disc < 0.0f;
-  // inside: the furthest non-negative root (:96-115)
-  const float f_in = miss ? 0.0f : MAXr(MAXr(fpos, fneg), 0.0f);
-  // outside: closest positive root, or the tolerance-band rules (:129-159)
-  float f_out = 0.0f;
-  f_out = ((fpos < 0.0f && fneg < 0.0f) || (b > 0.0f)) ? kLarge : f_out;
-  f_out = (fpos > 0.0f && fneg > 0.0f) ? MINr(fpos, fneg) : f_out;
-  f_out = miss ? kLarge : f_out;
-  int idx;
-  __builtin_memcpy(&idx, &B.w, sizeof(idx));
-  const bool use_in = relevant && inside;                      // :428-440
-  const bool take_in = use_in && (acc.fin < f_in);
-  acc.any_in = acc.any_in || use_in;
-  acc.fin = take_in ? f_in : acc.fin;
-  acc.kin = take_in ? idx : acc.kin;
-  const bool take_out = relevant && !inside && (A.w < in.sid) && (f_out < acc.fout);  // :441-454
-  acc.fout = take_out ? f_out : acc.fout;
-  acc.kout = take_out ? idx : acc.kout;
+  const float f_in = miss ? 0.0f : fmaxf(root, 0.0f);
+  const float f_out = (miss | away) ? kLarge : fmaxf(near, 0.0f);
+  const uint32_t key_in = (f2u(f_in) & kKeyMask) | pos1;
+  const uint32_t key_out = (f2u(f_out) & kKeyMask) | pos1;
+  const bool use_in = relevant & inside & !guard;                     // :428-440
+  const bool use_out = !inside & (A.w < in.sid) & !guard;             // :441-454
+  k.kin = (use_in & (key_in > k.kin)) ? key_in : k.kin;
+  k.kout = (use_out & (key_out < k.kout)) ? key_out : k.kout;
 }
 
 NZ_HD inline int cell_id(const Grid& g, int lx, int ly, int lz) {
@@ -241,20 +272,50 @@ NZ_HD inline bool rebin(const Grid& g, double p[3], int l[3], int u[3]) {
   return ok;
 }
 
-// The candidate loop, with the next candidate's loads issued before the
-// current one is evaluated.
-NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, ScanAcc& acc) {
-  if (cs >= ce) return;
-  float4 A = g.cand_a[cs];
-  float4 B = g.cand_b[cs];
-  for (int k = cs; k < ce; ++k) {
-    const int kn = (k + 1 < ce) ? k + 1 : k;
-    const float4 An = g.cand_a[kn];
-    const float4 Bn = g.cand_b[kn];
-    visit(in, A, B, acc);
-    A = An;
-    B = Bn;
+NZ_HD inline void keys_reset(const Grid& g, ScanKeys& k) {
+  k.kin = 0u;
+  k.kout = 0xffffffffu;
+  k.sid_new = (float)g.nstruct;
+  k.gcnt = 0;
+  k.gpos = 0u;
+}
+
+// Sphere index of list entry e (the B record's w bits).
+NZ_HD inline int cand_sphere(const Grid& g, int e) {
+  return (int)f2u(g.cand[2 * (int64_t)e + 1].w);
+}
+
+// The candidate loop over the list [cs, ce), with the next record's loads
+// issued before the current one is evaluated.  Reading one record past the
+// list is in bounds: the record array ends with a padding record.
+NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, ScanKeys& k) {
+  const int n = ce - cs;
+  const float4* rec = g.cand + 2 * (int64_t)cs;
+  float4 A0 = rec[0];
+  float4 B0 = rec[1];
+  int e = 0;
+  for (; e + 2 <= n; e += 2) {  // two records per trip, each loaded one visit ahead
+    const float4 A1 = rec[2];
+    const float4 B1 = rec[3];
+    visit(in, A0, B0, (uint32_t)(e + 1), k);
+    A0 = rec[4];
+    B0 = rec[5];
+    visit(in, A1, B1, (uint32_t)(e + 2), k);
+    rec += 4;
   }
+  if (e < n) visit(in, A0, B0, (uint32_t)(e + 1), k);
+}
+
+// The keys of a finished scan as the FP64 code's accumulator.
+NZ_HD inline void keys_decode(const Grid& g, int cs, const ScanKeys& k, ScanAcc& acc) {
+  acc.sid_new = k.sid_new;
+  acc.any_in = k.kin != 0u;
+  acc.fin = acc.any_in ? u2f(k.kin & kKeyMask) : 0.0f;
+  acc.kin = acc.any_in ? cand_sphere(g, cs + (int)(k.kin & 0xffu) - 1) : -1;
+  const bool any_out = k.kout != 0xffffffffu;
+  acc.fout = any_out ? u2f(k.kout & kKeyMask) : kLarge;
+  acc.kout = any_out ? cand_sphere(g, cs + (int)(k.kout & 0xffu) - 1) : -1;
+  acc.exact = false;
 }
 
 // The scan in FP64 (exact re-solve of a sub-step): over the lane's cell list
@@ -275,7 +336,7 @@ struct ExactGeom {
   const double4* sph;
   const int32_t* sph_sid;
   const int32_t* cell_start;
-  const float4* cand_b;
+  const float4* cand;
   int64_t nspheres;
   int32_t nloc0, nloc1;
   int32_t nstruct;
@@ -310,7 +371,7 @@ NZ_HD NZ_EXACT_ATTR ScanAcc scan_exact_call(ExactGeom eg, int mode, int cs, int 
         }
         for (int q = b; q < e; ++q) {
           int k;
-          const float w = eg.cand_b[q].w;
+          const float w = eg.cand[2 * (int64_t)q + 1].w;
           __builtin_memcpy(&k, &w, sizeof(k));
           visit_exact(g, p, j, lo, sid, k, acc);
         }
@@ -324,7 +385,7 @@ NZ_HD inline void scan_exact(const Grid& g, int mode, int cs, int ce, const doub
   eg.sph = g.sph;
   eg.sph_sid = g.sph_sid;
   eg.cell_start = g.cell_start;
-  eg.cand_b = g.cand_b;
+  eg.cand = g.cand;
   eg.nspheres = g.nspheres;
   eg.nloc0 = g.nloc[0];
   eg.nloc1 = g.nloc[1];
@@ -435,11 +496,23 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
         in.p[a] = (float)s.p[a];
         in.j[a] = (float)q.j[a];
       }
-      in.a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
-      in.inv_a = 1.0f / in.a;
+      const float a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
+      in.a4 = 4.0f * a;
+      in.inv_a = 1.0f / a;
       in.sid = s.sid;
-      scan_list(g, q.cs, q.ce, in, acc);
-      if (acc.exact) exact_mode = kExactList;
+      ScanKeys k;
+      keys_reset(g, k);
+      scan_list(g, q.cs, q.ce, in, k);
+      if (k.gcnt > 1) {
+        exact_mode = kExactList;
+      } else {
+        keys_decode(g, q.cs, k, acc);
+        if (k.gcnt == 1) {  // the one undecided candidate in FP64
+          s.nguard++;
+          const double lo[3] = {cell_lo(g, 0, s.l[0]), cell_lo(g, 1, s.l[1]), cell_lo(g, 2, s.l[2])};
+          visit_exact(g, s.p, q.j, lo, s.sid, cand_sphere(g, q.cs + (int)k.gpos - 1), acc);
+        }
+      }
     }
     if (exact_mode >= 0) {  // rare: the scan in FP64
       s.nguard++;
@@ -561,9 +634,7 @@ NZ_HD inline void lane_init(const Grid& g, const float pos[3], Lane& s) {
   const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
   float sid = (float)g.nstruct;
   for (int k = g.cell_start[cid]; k < g.cell_start[cid + 1]; ++k) {
-    int idx;
-    const float4 B = g.cand_b[k];
-    __builtin_memcpy(&idx, &B.w, sizeof(idx));
+    const int idx = cand_sphere(g, k);
     const double4 c = g.sph[idx];
     const double dx = s.p[0] - (c.x - cell_lo(g, 0, s.l[0]));
     const double dy = s.p[1] - (c.y - cell_lo(g, 1, s.l[1]));
