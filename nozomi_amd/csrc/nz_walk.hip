@@ -37,6 +37,7 @@ struct Walker {
   int32_t* keys = nullptr;    // [n] sort keys
   int32_t* counts = nullptr;  // [nkeys + 1]
   int32_t* cursor = nullptr;  // [nkeys]
+  int32_t* partial = nullptr; // [scan tiles] tile sums of the key-count scan
   float* pos_tmp = nullptr;   // [3,n] seeding output
   int64_t nkeys = 0;          // sort key range: 2 x cells (intra / extra)
   int32_t sort_interval = 64;
@@ -290,29 +291,97 @@ __global__ void __launch_bounds__(256) sort_keys_kernel(const Grid g, const Slot
   if (valid && r.head == (int)(threadIdx.x & 63)) atomicAdd(&counts[k], r.len);
 }
 
-// Exclusive scan of counts[0..m) in place (counts[m] = total), one workgroup.
-__global__ void __launch_bounds__(1024) scan_kernel(int32_t* counts, int64_t m) {
-  __shared__ int32_t part[1024];
-  const int t = threadIdx.x;
-  const int64_t per = (m + 1023) / 1024;
-  const int64_t b = t * per, e = min<int64_t>(m, b + per);
-  int32_t s = 0;
-  for (int64_t k = b; k < e; ++k) s += counts[k];
-  part[t] = s;
+// Exclusive scan of the m key counts in place, three coalesced passes over
+// tiles of kScanTile: tile sums, a scan of the tile sums, and the scan of each
+// tile from its offset.  The scan runs over m + 1 entries whose last one is 0,
+// so counts[m] ends up holding the total.
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 8;
+constexpr int kScanTile = kScanThreads * kScanPer;
+
+// Exclusive scan of one value per thread across a 256-thread block; *total
+// receives the block sum.
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* total) {
+  __shared__ int32_t wsum[kScanThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-    const int32_t v = (t >= off) ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  int32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; ++w) {
+    before += (w < wave) ? wsum[w] : 0;
+    all += wsum[w];
   }
-  int32_t run = (t == 0) ? 0 : part[t - 1];
-  for (int64_t k = b; k < e; ++k) {
-    const int32_t c = counts[k];
-    counts[k] = run;
-    run += c;
+  __syncthreads();  // wsum may be reused by the caller's next scan
+  *total = all;
+  return before + x - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) tile_sum_kernel(const int32_t* counts, int64_t m,
+                                                                int32_t* partial) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t i = base + k * kScanThreads + threadIdx.x;
+    s += (i < m) ? counts[i] : 0;
   }
-  if (t == 1023) counts[m] = part[1023];
+  int32_t total;
+  (void)block_exclusive_scan(s, &total);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// Exclusive scan of the nb tile sums, one block, in chunks of 256.
+__global__ void __launch_bounds__(kScanThreads) partial_scan_kernel(int32_t* partial, int64_t nb) {
+  int32_t carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += kScanThreads) {
+    const int64_t i = b0 + threadIdx.x;
+    const int32_t v = (i < nb) ? partial[i] : 0;
+    int32_t total;
+    const int32_t ex = block_exclusive_scan(v, &total);
+    if (i < nb) partial[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads) tile_scan_kernel(int32_t* counts, int64_t m,
+                                                                 const int32_t* partial) {
+  __shared__ int32_t tile[kScanTile + kScanTile / 32];  // +1 word per 32: fewer bank conflicts
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  auto slot = [](int k) { return k + (k >> 5); };
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    const int e = k * kScanThreads + threadIdx.x;
+    const int64_t i = base + e;
+    tile[slot(e)] = (i < m) ? counts[i] : 0;
+  }
+  __syncthreads();
+  int32_t v[kScanPer], s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = tile[slot(threadIdx.x * kScanPer + k)];
+    s += v[k];
+  }
+  int32_t total;
+  int32_t run = block_exclusive_scan(s, &total) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    tile[slot(threadIdx.x * kScanPer + k)] = run;
+    run += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    const int e = k * kScanThreads + threadIdx.x;
+    const int64_t i = base + e;
+    if (i < m) counts[i] = tile[slot(e)];
+  }
 }
 
 __global__ void __launch_bounds__(256) sort_scatter_kernel(const SlotState src, const SlotState dst,
@@ -449,7 +518,12 @@ static int sort_spins(Walker* w, hipStream_t stream) {
                      w->nkeys, 0);
   hipLaunchKernelGGL(sort_keys_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, g, w->buf[w->cur],
                      n, w->keys, w->counts);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, w->counts, w->nkeys);
+  const int64_t m = w->nkeys + 1, nb = (m + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(tile_sum_kernel, dim3(nb), dim3(kScanThreads), 0, stream, w->counts, m,
+                     w->partial);
+  hipLaunchKernelGGL(partial_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, w->partial, nb);
+  hipLaunchKernelGGL(tile_scan_kernel, dim3(nb), dim3(kScanThreads), 0, stream, w->counts, m,
+                     w->partial);
   hipLaunchKernelGGL(sort_scatter_kernel, dim3(blocks_for(n)), dim3(256), 0, stream,
                      w->buf[w->cur], w->buf[1 - w->cur], n, w->keys, w->counts, w->cursor);
   NZ_HIP(hipGetLastError());
@@ -516,7 +590,11 @@ int nz_walker_create(nz_substrate_t sub, int64_t n, uint32_t flags, nz_walker_t*
   if (!rc) rc = walker_alloc(w, sizeof(int32_t) * n, &p);
   if (!rc) { w->keys = (int32_t*)p; rc = walker_alloc(w, sizeof(int32_t) * (w->nkeys + 1), &p); }
   if (!rc) { w->counts = (int32_t*)p; rc = walker_alloc(w, sizeof(int32_t) * w->nkeys, &p); }
-  if (!rc) { w->cursor = (int32_t*)p; rc = walker_alloc(w, sizeof(float) * 3 * n, &p); }
+  if (!rc) {
+    w->cursor = (int32_t*)p;
+    rc = walker_alloc(w, sizeof(int32_t) * ((w->nkeys + 1 + kScanTile - 1) / kScanTile), &p);
+  }
+  if (!rc) { w->partial = (int32_t*)p; rc = walker_alloc(w, sizeof(float) * 3 * n, &p); }
   if (!rc) w->pos_tmp = (float*)p;
   if (rc) {
     walker_release(w);
