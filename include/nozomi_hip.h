@@ -158,6 +158,47 @@ int nz_displacement_moments(nz_walker_t w, double* out, void* stream);
 int nz_walker_time_kernels(nz_walker_t w, int32_t enable);
 int nz_walker_kernel_time(nz_walker_t w, double* ms_total, int64_t* launches);
 
+/* ---- many simulations in one launch (no counterpart in the reference) ----
+ * The batched and served forms of the walker: a pool of spin slots shared by
+ * up to max_sims independent simulations, each with its own substrate, spin
+ * count, Philox key, time step, length and phase coefficients.  One launch
+ * advances every active simulation by up to its k steps; a spin's trajectory
+ * is the one nz_walk gives for the same substrate, key and start position
+ * (counter-based RNG), whatever else shares the launch.  The sort/walk/advance
+ * sequence runs on the batch's own stream, captured in a HIP graph, ordered
+ * after and before the caller's stream by events. */
+typedef struct nz_batch* nz_batch_t;
+#define NZ_EFULL (-2)   /* nz_batch_add: no free simulation slot or spin blocks */
+
+/* capacity: spin slots (rounded up to 256); max_sims: concurrent simulations. */
+int nz_batch_create(int64_t capacity, int32_t max_sims, nz_batch_t* out);
+int nz_batch_destroy(nz_batch_t b);
+/* Add a simulation of n spins on sub: nsteps steps of dt (ms) with Philox key
+ * seed, optional per-step phase coefficients g/garea (device float [nsteps],
+ * kept by the caller until the simulation is removed), NZ_WALK_* flags, k
+ * steps per launch.  Start positions: pos (device [3,n]) or, when pos is NULL,
+ * rejection seeding into targets_host with key seed ^ 0x5EED5EED5EED (as
+ * DiffSim3d.setup).  Returns NZ_EFULL when the pool has no room. */
+int nz_batch_add(nz_batch_t b, nz_substrate_t sub, int64_t n, uint64_t seed, int64_t nsteps,
+                 double dt, const float* g, const float* garea, uint32_t flags, int32_t k,
+                 const float* pos, const int32_t* targets_host, int32_t ntargets, int32_t* sim,
+                 void* stream);
+int nz_batch_remove(nz_batch_t b, int32_t sim);
+/* Enqueue `launches` launches (pairs replay the captured graph). */
+int nz_batch_run(nz_batch_t b, int32_t launches, void* stream);
+int nz_batch_set_graph(nz_batch_t b, int32_t enable);
+/* Steps completed and whether the simulation finished (waits for the batch). */
+int nz_batch_status(nz_batch_t b, int32_t sim, int64_t* step, int32_t* done);
+/* Free capacity: largest contiguous run of free slots, all free slots, free
+ * simulation entries. */
+int nz_batch_free(nz_batch_t b, int64_t* largest_run, int64_t* free_slots, int32_t* free_sims);
+/* A simulation's state in spin order (as nz_walker_read). */
+int nz_batch_read(nz_batch_t b, int32_t sim, float* pos, float* pos0, float* sig, float* mom,
+                  void* stream);
+/* A simulation's DWI signal sums (as nz_signal_reduce over its moments). */
+int nz_batch_signals(nz_batch_t b, int32_t sim, int32_t W, const float* dirs, const float* scale,
+                     double* out, void* stream);
+
 /* Phases of W directions from the moments: phase[w, i] = scale[w] *
  * (dirs[0,w] m_x + dirs[1,w] m_y + dirs[2,w] m_z).  Replaces the per-step
  * W-wide accumulation of randomWalk3d_phase_multidirr (sim3d_kernel.cu:880-942);

@@ -67,6 +67,22 @@ _SIGS = {
     "nz_walker_counters": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp]),
     "nz_displacement_moments": (ctypes.c_int, [_c_vp, _c_vp, _c_vp]),
     "nz_walker_time_kernels": (ctypes.c_int, [_c_vp, ctypes.c_int32]),
+    "nz_batch_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_c_vp)]),
+    "nz_batch_destroy": (ctypes.c_int, [_c_vp]),
+    "nz_batch_add": (ctypes.c_int, [_c_vp, _c_vp, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int64,
+                                    ctypes.c_double, _c_vp, _c_vp, ctypes.c_uint32, ctypes.c_int32,
+                                    _c_vp, _c_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                    _c_vp]),
+    "nz_batch_remove": (ctypes.c_int, [_c_vp, ctypes.c_int32]),
+    "nz_batch_run": (ctypes.c_int, [_c_vp, ctypes.c_int32, _c_vp]),
+    "nz_batch_set_graph": (ctypes.c_int, [_c_vp, ctypes.c_int32]),
+    "nz_batch_status": (ctypes.c_int, [_c_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int32)]),
+    "nz_batch_free": (ctypes.c_int, [_c_vp, ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
+    "nz_batch_read": (ctypes.c_int, [_c_vp, ctypes.c_int32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "nz_batch_signals": (ctypes.c_int, [_c_vp, ctypes.c_int32, ctypes.c_int32, _c_vp, _c_vp, _c_vp,
+                                        _c_vp]),
     "nz_walker_kernel_time": (ctypes.c_int, [_c_vp, ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_int64)]),
     "nz_project_phase": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_vp, _c_vp, _c_vp,

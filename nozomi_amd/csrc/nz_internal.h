@@ -63,6 +63,11 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
+// Sum over the n entries of mom [3, ld] (entries with valid[i] < 0 skipped)
+// of cos / sin of scale[w] * dirs[:, w] . m, into out [2W] (FP64).
+int signal_reduce(int64_t n, int64_t ld, const int32_t* valid, int32_t W, const float* mom,
+                  const float* dirs, const float* scale, double* out, hipStream_t stream);
+
 }  // namespace nz
 
 #define NZ_HIP(call)                                           \

@@ -29,8 +29,10 @@ constexpr int kSigTile = 256;   // spins staged per LDS tile
 constexpr int kSigTiles = 8;    // tiles per block
 
 // Block: kSigTiles*kSigTile spins.  Thread t handles signals t, t+256, ...
-// reading every staged moment from LDS (broadcast reads).
-__global__ void __launch_bounds__(256) signal_kernel(int64_t n, int32_t W, const float* mom,
+// reading every staged moment from LDS (broadcast reads).  mom is [3, ld];
+// entries whose valid[i] < 0 (padding slots of a batch) count as absent.
+__global__ void __launch_bounds__(256) signal_kernel(int64_t n, int64_t ld, const int32_t* valid,
+                                                     int32_t W, const float* mom,
                                                      const float* dirs, const float* scale,
                                                      double* out) {
   __shared__ float sm[3][kSigTile];
@@ -53,11 +55,28 @@ __global__ void __launch_bounds__(256) signal_kernel(int64_t n, int32_t W, const
     if (t0 >= n) break;
     const int64_t i = t0 + threadIdx.x;
     __syncthreads();
-    sm[0][threadIdx.x] = (i < n) ? mom[i] : 0.0f;
-    sm[1][threadIdx.x] = (i < n) ? mom[n + i] : 0.0f;
-    sm[2][threadIdx.x] = (i < n) ? mom[2 * n + i] : 0.0f;
+    const bool ok = i < n && (!valid || valid[i] >= 0);
+    const float v0 = ok ? mom[i] : 0.0f;
+    const float v1 = ok ? mom[ld + i] : 0.0f;
+    const float v2 = ok ? mom[2 * ld + i] : 0.0f;
+    // the present entries are packed to the front of the tile (ballot + count)
+    const unsigned long long m = __ballot(ok);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ int wcnt[kSigTile / 64];
+    if (lane == 0) wcnt[wave] = __popcll(m);
     __syncthreads();
-    const int cnt = (int)min<int64_t>(kSigTile, n - t0);
+    int before = 0, cnt = 0;
+    for (int q = 0; q < kSigTile / 64; ++q) {
+      before += (q < wave) ? wcnt[q] : 0;
+      cnt += wcnt[q];
+    }
+    if (ok) {
+      const int dst = before + __popcll(m & ((1ull << lane) - 1));
+      sm[0][dst] = v0;
+      sm[1][dst] = v1;
+      sm[2][dst] = v2;
+    }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < kMaxPer; ++q) {
       if ((int)threadIdx.x + q * 256 >= W) break;
@@ -94,6 +113,20 @@ __global__ void zero_f64(double* p, int64_t n) {
   if (i < n) p[i] = 0.0;
 }
 
+int signal_reduce(int64_t n, int64_t ld, const int32_t* valid, int32_t W, const float* mom,
+                  const float* dirs, const float* scale, double* out, hipStream_t stream) {
+  if (n <= 0 || W <= 0 || W > 2048) return fail(-1, "signal reduction: need n > 0 and 0 < W <= 2048");
+  if (!mom || !dirs || !out) return fail(-1, "signal reduction: NULL buffer");
+  // a fill kernel rather than hipMemsetAsync, so the call can be graph-captured
+  hipLaunchKernelGGL(zero_f64, dim3((unsigned)((2 * W + 255) / 256)), dim3(256), 0, stream, out,
+                     (int64_t)2 * W);
+  const int64_t per = (int64_t)kSigTile * kSigTiles;
+  hipLaunchKernelGGL(signal_kernel, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0, stream, n,
+                     ld, valid, W, mom, dirs, scale, out);
+  NZ_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace nz
 
 using namespace nz;
@@ -114,15 +147,7 @@ int nz_project_phase(int64_t n, int32_t W, const float* mom, const float* dirs, 
 int nz_signal_reduce(int64_t n, int32_t W, const float* mom, const float* dirs, const float* scale,
                      double* out, void* stream) {
   if (n <= 0 || W <= 0 || W > 2048) return fail(-1, "nz_signal_reduce: need n > 0 and 0 < W <= 2048");
-  if (!mom || !dirs || !out) return fail(-1, "nz_signal_reduce: NULL buffer");
-  // a fill kernel rather than hipMemsetAsync, so the call can be graph-captured
-  hipLaunchKernelGGL(zero_f64, dim3((unsigned)((2 * W + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, out, (int64_t)2 * W);
-  const int64_t per = (int64_t)kSigTile * kSigTiles;
-  hipLaunchKernelGGL(signal_kernel, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0,
-                     (hipStream_t)stream, n, W, mom, dirs, scale, out);
-  NZ_HIP(hipGetLastError());
-  return 0;
+  return signal_reduce(n, n, nullptr, W, mom, dirs, scale, out, (hipStream_t)stream);
 }
 
 }  // extern "C"

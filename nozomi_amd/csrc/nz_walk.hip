@@ -115,40 +115,51 @@ constexpr int kMomChunk = 256;
 #define NZ_WALK_ATTR
 #endif
 
-__global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A) {
-  __shared__ float smom[kMomChunk * 6];
-  const Grid& g = A.g;
-  const SlotState& st = A.st;
-  const int64_t n = A.n;
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
-  const int64_t i = blk * blockDim.x + threadIdx.x;
-  const bool active = i < n;
-  const int64_t ii = active ? i : 0;
+// Per-launch parameters of one simulation's walk.
+struct StepParams {
+  uint64_t seed;
+  int64_t base;      // global step of gph[0] / garea[0]
+  int64_t target;    // the wavefront runs until all its spins reached this step
+  int64_t limit;     // a spin may run ahead up to this step
+  int64_t mom_base;  // global step of stepmom[0] for this launch
+  double dt;
+  const float* gph;
+  const float* garea;
+  double* stepmom;   // per-step moment sums (needs smom and target - mom_base <= kMomChunk)
+  uint32_t flags;
+};
 
+// Walk the spin held in slot i (fields strided by ld) from its next step to
+// at least P.target.  Every lane of the block calls it (inactive lanes too:
+// the per-step moment partials synchronise the block).
+__device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, int64_t ld,
+                                          int64_t i, bool active, const StepParams P,
+                                          float* smom) {
+  const int64_t ii = active ? i : 0;
   Lane s;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    s.u[a] = active ? st.ucell[a * n + ii] : 0;
-    s.p[a] = active ? st.rel[a * n + ii] : 0.0;
-    s.m[a] = (active && A.gph) ? st.mom[a * n + ii] : 0.0f;
+    s.u[a] = active ? st.ucell[a * ld + ii] : 0;
+    s.p[a] = active ? st.rel[a * ld + ii] : 0.0;
+    s.m[a] = (active && P.gph) ? st.mom[a * ld + ii] : 0.0f;
   }
   lane_load_cells(g, s);
   s.logsig = active ? st.logsig[ii] : 0.0f;
   s.sid = active ? (float)st.sid[ii] : (float)g.nstruct;
   s.nsub = 0;
   s.nguard = 0;
-  const bool do_t2 = (A.flags & NZ_WALK_T2) != 0;
+  const bool do_t2 = (P.flags & NZ_WALK_T2) != 0;
   const uint32_t spin = active ? (uint32_t)st.spin_id[ii] : 0u;
-  int64_t t = active ? (int64_t)st.tstep[ii] : A.target;
+  int64_t t = active ? (int64_t)st.tstep[ii] : P.target;
   int u0[3] = {0, 0, 0};
   double p0[3] = {0.0, 0.0, 0.0};
-  if (A.stepmom) {
-    for (int k = threadIdx.x; k < (int)(A.target - A.mom_base) * 6; k += blockDim.x) smom[k] = 0.0f;
+  if (P.stepmom) {
+    for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x) smom[k] = 0.0f;
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      u0[a] = active ? st.ucell0[a * n + ii] : 0;
-      p0[a] = active ? st.rel0[a * n + ii] : 0.0;
+      u0[a] = active ? st.ucell0[a * ld + ii] : 0;
+      p0[a] = active ? st.rel0[a * ld + ii] : 0.0;
     }
   }
 
@@ -157,21 +168,21 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A
   q.pending = -1;
   // A spin only stops at a step boundary: the sub-step state of a step in
   // progress is not kept across launches.
-  while (__any(t < A.target || started)) {
-    const bool behind = __any(t < A.target);
-    if (active && (started || (t < A.limit && (t < A.target || behind)))) {
+  while (__any(t < P.target || started)) {
+    const bool behind = __any(t < P.target);
+    if (active && (started || (t < P.limit && (t < P.target || behind)))) {
       const uint32_t step = (uint32_t)t;
       if (!started) {
-        step_begin(g, s, q, A.dt);
+        step_begin(g, s, q, P.dt);
         started = true;
       }
-      if (q.pending >= 0) step_draw(q, A.seed, spin, step);
-      if (substep(g, s, q, A.seed, spin, step, A.dt, do_t2)) {
-        const float G = A.gph ? A.gph[t - A.base] : 0.0f;
-        const float Ga = A.gph ? A.garea[t - A.base] : 0.0f;
+      if (q.pending >= 0) step_draw(q, P.seed, spin, step);
+      if (substep(g, s, q, P.seed, spin, step, P.dt, do_t2)) {
+        const float G = P.gph ? P.gph[t - P.base] : 0.0f;
+        const float Ga = P.gph ? P.garea[t - P.base] : 0.0f;
         step_end(g, s, G, Ga);
-        if (A.stepmom) {  // moment sums (sim3d_kernel.cu:191-237)
-          const int k = (int)(t - A.mom_base) * 6;
+        if (P.stepmom) {  // moment sums (sim3d_kernel.cu:191-237)
+          const int k = (int)(t - P.mom_base) * 6;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
             const float d = (float)lane_disp(g, s, u0, p0, a);
@@ -184,24 +195,33 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A
       }
     }
   }
-  if (A.stepmom) {  // block partials -> FP64 sums
+  if (P.stepmom) {  // block partials -> FP64 sums
     __syncthreads();
-    for (int k = threadIdx.x; k < (int)(A.target - A.mom_base) * 6; k += blockDim.x)
-      atomicAdd(&A.stepmom[(A.mom_base - A.base) * 6 + k], (double)smom[k]);
+    for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x)
+      atomicAdd(&P.stepmom[(P.mom_base - P.base) * 6 + k], (double)smom[k]);
   }
 
   if (!active) return;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    st.ucell[a * n + i] = s.u[a];
-    st.rel[a * n + i] = s.p[a];
-    if (A.gph) st.mom[a * n + i] = s.m[a];
+    st.ucell[a * ld + i] = s.u[a];
+    st.rel[a * ld + i] = s.p[a];
+    if (P.gph) st.mom[a * ld + i] = s.m[a];
   }
   st.logsig[i] = s.logsig;
   st.sid[i] = (int32_t)s.sid;
   st.tstep[i] = (uint32_t)t;
   if (st.nsub) st.nsub[i] += s.nsub;
   if (st.nguard) st.nguard[i] += s.nguard;
+}
+
+__global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A) {
+  __shared__ float smom[kMomChunk * 6];
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  const StepParams P{A.seed, A.base, A.target, A.limit, A.mom_base, A.dt,
+                     A.gph,  A.garea, A.stepmom, A.flags};
+  walk_slot(A.g, A.st, A.n, i, i < A.n, P, smom);
 }
 
 // Absolute start positions (spin order) -> resident state (slot = spin).
@@ -323,9 +343,18 @@ __device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* tota
   return before + x - v;
 }
 
+// With nkeys_dev the scan length is *nkeys_dev + 1 (a batch's key space,
+// which changes with its membership) and surplus blocks return.
+__device__ __forceinline__ int64_t scan_len(int64_t m, const int64_t* nkeys_dev) {
+  return nkeys_dev ? *nkeys_dev + 1 : m;
+}
+
 __global__ void __launch_bounds__(kScanThreads) tile_sum_kernel(const int32_t* counts, int64_t m,
-                                                                int32_t* partial) {
+                                                                int32_t* partial,
+                                                                const int64_t* nkeys_dev) {
+  m = scan_len(m, nkeys_dev);
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  if (base >= m) return;
   int32_t s = 0;
 #pragma unroll
   for (int k = 0; k < kScanPer; ++k) {
@@ -338,7 +367,9 @@ __global__ void __launch_bounds__(kScanThreads) tile_sum_kernel(const int32_t* c
 }
 
 // Exclusive scan of the nb tile sums, one block, in chunks of 256.
-__global__ void __launch_bounds__(kScanThreads) partial_scan_kernel(int32_t* partial, int64_t nb) {
+__global__ void __launch_bounds__(kScanThreads) partial_scan_kernel(int32_t* partial, int64_t nb,
+                                                                    const int64_t* nkeys_dev) {
+  if (nkeys_dev) nb = (*nkeys_dev + 1 + kScanTile - 1) / kScanTile;
   int32_t carry = 0;
   for (int64_t b0 = 0; b0 < nb; b0 += kScanThreads) {
     const int64_t i = b0 + threadIdx.x;
@@ -351,9 +382,12 @@ __global__ void __launch_bounds__(kScanThreads) partial_scan_kernel(int32_t* par
 }
 
 __global__ void __launch_bounds__(kScanThreads) tile_scan_kernel(int32_t* counts, int64_t m,
-                                                                 const int32_t* partial) {
+                                                                 const int32_t* partial,
+                                                                 const int64_t* nkeys_dev) {
   __shared__ int32_t tile[kScanTile + kScanTile / 32];  // +1 word per 32: fewer bank conflicts
+  m = scan_len(m, nkeys_dev);
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  if (base >= m) return;
   auto slot = [](int k) { return k + (k >> 5); };
 #pragma unroll
   for (int k = 0; k < kScanPer; ++k) {
@@ -414,23 +448,25 @@ __global__ void __launch_bounds__(256) sort_scatter_kernel(const SlotState src, 
 }
 
 // ---- read-back in spin order ---------------------------------------------
-__global__ void __launch_bounds__(256) read_kernel(const Grid g, const SlotState st, int64_t n,
-                                                   float* pos, float* pos0, float* sig, float* mom,
+__global__ void __launch_bounds__(256) read_kernel(const Grid g, const SlotState st, int64_t ld,
+                                                   int64_t nslots, int64_t n, float* pos,
+                                                   float* pos0, float* sig, float* mom,
                                                    double* pos_d, double* disp_d, int32_t* nsub,
                                                    int32_t* nguard) {
+  // slots [0, nslots) of st (fields strided by ld); outputs [.., n] in spin order
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= nslots) return;
   const int64_t k = st.spin_id[i];
-  if (k < 0 || k >= n) return;  // every slot holds a spin id in [0, n)
+  if (k < 0 || k >= n) return;  // padding slots hold -1
   Lane s;
   int u0[3];
   double p0[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    s.u[a] = st.ucell[a * n + i];
-    s.p[a] = st.rel[a * n + i];
-    u0[a] = st.ucell0[a * n + i];
-    p0[a] = st.rel0[a * n + i];
+    s.u[a] = st.ucell[a * ld + i];
+    s.p[a] = st.rel[a * ld + i];
+    u0[a] = st.ucell0[a * ld + i];
+    p0[a] = st.rel0[a * ld + i];
   }
   lane_load_cells(g, s);
   if (pos || pos0) {
@@ -445,7 +481,7 @@ __global__ void __launch_bounds__(256) read_kernel(const Grid g, const SlotState
   if (sig) sig[k] = expf(st.logsig[i]);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    if (mom) mom[a * n + k] = st.mom[a * n + i];
+    if (mom) mom[a * n + k] = st.mom[a * ld + i];
     if (pos_d) pos_d[a * n + k] = cell_lo(g, a, s.l[a]) + s.p[a];
     if (disp_d) disp_d[a * n + k] = lane_disp(g, s, u0, p0, a);
   }
@@ -520,10 +556,11 @@ static int sort_spins(Walker* w, hipStream_t stream) {
                      n, w->keys, w->counts);
   const int64_t m = w->nkeys + 1, nb = (m + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(tile_sum_kernel, dim3(nb), dim3(kScanThreads), 0, stream, w->counts, m,
-                     w->partial);
-  hipLaunchKernelGGL(partial_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, w->partial, nb);
+                     w->partial, (const int64_t*)nullptr);
+  hipLaunchKernelGGL(partial_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, w->partial, nb,
+                     (const int64_t*)nullptr);
   hipLaunchKernelGGL(tile_scan_kernel, dim3(nb), dim3(kScanThreads), 0, stream, w->counts, m,
-                     w->partial);
+                     w->partial, (const int64_t*)nullptr);
   hipLaunchKernelGGL(sort_scatter_kernel, dim3(blocks_for(n)), dim3(256), 0, stream,
                      w->buf[w->cur], w->buf[1 - w->cur], n, w->keys, w->counts, w->cursor);
   NZ_HIP(hipGetLastError());
@@ -535,6 +572,316 @@ static int sort_spins(Walker* w, hipStream_t stream) {
 static int check_walker(const Walker* w, const char* who) {
   if (!w) return fail(-1, std::string(who) + ": walker is NULL");
   return 0;
+}
+
+// ---- many simulations in one launch ---------------------------------------
+// A batch holds a pool of slots (256 per block) shared by up to max_sims
+// simulations, each with its own substrate, spin count, Philox key, time step,
+// length and phase coefficients.  A simulation owns a contiguous range of
+// whole blocks, so every workgroup walks exactly one simulation and reads its
+// descriptor with uniform loads; its padding slots hold spin id -1.  One
+// launch advances every active simulation by up to k steps; spatial sorting
+// keeps each simulation's spins inside its range (the key space is laid out
+// in pool order: a simulation's class/cell keys and one padding key, one key
+// per free gap), so the walk kernel, sort and scan are those of the single
+// walker.  The launch sequence (sort, walk, advance) depends on no host value
+// that membership changes, so it is captured once in a HIP graph and replayed.
+
+constexpr int64_t kBlk = 256;
+
+struct SimDesc {
+  Grid g;
+  uint64_t seed;
+  double dt;
+  const float* gph;
+  const float* garea;
+  uint32_t flags;
+  int32_t active;   // 1 while the simulation walks
+  int64_t nsteps;   // walk length
+  int64_t step;     // steps every spin of the simulation has completed
+  int64_t k;        // steps per launch
+  int64_t key0;     // first sort key
+  int64_t ncells;   // cells of its grid
+};
+
+struct BatchJob {
+  bool used = false;
+  const Substrate* sub = nullptr;
+  int64_t n = 0, b0 = 0, nb = 0;
+};
+
+struct Batch {
+  int64_t cap = 0, nblocks = 0;
+  int32_t max_sims = 0;
+  SlotState buf[2];
+  int cur = 0;
+  int32_t* keys = nullptr;
+  int32_t* counts = nullptr;
+  int32_t* cursor = nullptr;
+  int32_t* partial = nullptr;
+  int64_t max_keys = 0;        // allocated key space
+  int64_t* nkeys_dev = nullptr;
+  SimDesc* descs_dev = nullptr;
+  int32_t* block_sim_dev = nullptr;
+  int64_t* block_key_dev = nullptr;
+  float* pos_tmp = nullptr;    // [3, cap] seeding output
+  int32_t* nfailed_dev = nullptr;
+  std::vector<SimDesc> descs;
+  std::vector<int32_t> block_sim;
+  std::vector<int64_t> block_key;
+  std::vector<BatchJob> jobs;
+  bool dirty = true;           // host tables newer than the device's
+  int32_t xcd_chunk = 64;
+  hipStream_t stream = nullptr;  // own stream: launches and graph capture
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  bool use_graph = true;
+  hipGraphExec_t graph = nullptr;
+  int graph_cur = 0;           // slot buffer the captured pair of launches starts from
+  std::vector<void*> allocations;
+  std::vector<void*> key_allocations;
+  int device = 0;
+};
+
+__global__ void fill_i32_dev(int32_t* p, const int64_t* nkeys_dev, int64_t extra, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < *nkeys_dev + extra) p[i] = v;
+}
+
+// Slots [slot0, slot0 + nslots) of a simulation from its n start positions
+// [3, n]; padding slots get spin id -1.
+__global__ void __launch_bounds__(256) batch_init_kernel(const Grid g, const SlotState st,
+                                                         int64_t ld, int64_t slot0, int64_t nslots,
+                                                         int64_t n, const float* pos) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nslots) return;
+  const int64_t i = slot0 + e;
+  if (e >= n) {
+    st.spin_id[i] = -1;
+    st.tstep[i] = 0u;
+    return;
+  }
+  Lane s;
+  const float p[3] = {pos[e], pos[n + e], pos[2 * n + e]};
+  lane_init(g, p, s);
+  st.spin_id[i] = (int32_t)e;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    st.ucell[a * ld + i] = s.u[a];
+    st.ucell0[a * ld + i] = s.u[a];
+    st.rel[a * ld + i] = s.p[a];
+    st.rel0[a * ld + i] = s.p[a];
+    st.mom[a * ld + i] = 0.0f;
+  }
+  st.sid[i] = (int32_t)s.sid;
+  st.logsig[i] = 0.0f;
+  st.tstep[i] = 0u;
+}
+
+__global__ void __launch_bounds__(256) batch_keys_kernel(const SimDesc* descs,
+                                                         const int32_t* block_sim,
+                                                         const int64_t* block_key,
+                                                         const SlotState st, int64_t ld,
+                                                         int32_t* keys, int32_t* counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ld) return;
+  const int64_t pb = i / kBlk;
+  const int32_t s = block_sim[pb];
+  int64_t key;
+  if (s < 0) {
+    key = block_key[pb];
+  } else {
+    const SimDesc& d = descs[s];
+    const Grid& g = d.g;
+    if (st.spin_id[i] < 0) {
+      key = d.key0 + 2 * d.ncells;
+    } else {
+      int c[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int w = st.ucell[a * ld + i] % g.ncell[a];
+        c[a] = w < 0 ? w + g.ncell[a] : w;
+      }
+      const int64_t cell = ((int64_t)c[2] * g.ncell[1] + c[1]) * g.ncell[0] + c[0];
+      key = d.key0 + cell + (st.sid[i] >= g.nstruct ? d.ncells : 0);
+    }
+  }
+  keys[i] = (int32_t)key;
+  const WaveRun r = wave_run((int32_t)key, true);
+  if (r.head == (int)(threadIdx.x & 63)) atomicAdd(&counts[key], r.len);
+}
+
+__global__ void __launch_bounds__(256) NZ_WALK_ATTR batch_walk_kernel(const SimDesc* descs,
+                                                                      const int32_t* block_sim,
+                                                                      const SlotState st,
+                                                                      int64_t ld, int32_t chunk) {
+  const int64_t pb = xcd_block(blockIdx.x, gridDim.x, chunk);
+  const int32_t s = block_sim[pb];
+  if (s < 0) return;
+  const SimDesc& d = descs[s];
+  if (!d.active) return;
+  const int64_t i = pb * kBlk + threadIdx.x;
+  const int64_t target = min(d.step + d.k, d.nsteps);
+  const StepParams P{d.seed, 0, target, d.nsteps, 0, d.dt, d.gph, d.garea, nullptr, d.flags};
+  walk_slot(d.g, st, ld, i, st.spin_id[i] >= 0, P, nullptr);
+}
+
+// After a launch: every active simulation has advanced to its target.
+__global__ void batch_advance_kernel(SimDesc* descs, int32_t max_sims) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= max_sims) return;
+  SimDesc& d = descs[s];
+  if (!d.active) return;
+  d.step = min(d.step + d.k, d.nsteps);
+  if (d.step >= d.nsteps) d.active = 0;
+}
+
+static int batch_alloc(Batch* b, size_t bytes, void** out, std::vector<void*>& list) {
+  void* p = nullptr;
+  NZ_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+  list.push_back(p);
+  *out = p;
+  return 0;
+}
+
+static void batch_release(Batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  for (void* p : b->key_allocations) (void)hipFree(p);
+  b->key_allocations.clear();
+  for (void* p : b->allocations) (void)hipFree(p);
+  b->allocations.clear();
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->ev_in = b->ev_out = nullptr;
+  b->stream = nullptr;
+}
+
+// Key space and per-block tables from the membership (pool order).
+static int64_t batch_layout(Batch* b) {
+  int64_t key = 0;
+  int64_t pb = 0;
+  while (pb < b->nblocks) {
+    const int32_t s = b->block_sim[pb];
+    if (s >= 0) {
+      SimDesc& d = b->descs[s];
+      d.key0 = key;
+      key += 2 * d.ncells + 1;
+      const int64_t e = pb + b->jobs[s].nb;
+      for (; pb < e; ++pb) b->block_key[pb] = d.key0;
+    } else {
+      for (; pb < b->nblocks && b->block_sim[pb] < 0; ++pb) b->block_key[pb] = key;
+      ++key;
+    }
+  }
+  return key;
+}
+
+// Grow the key arrays to hold nkeys keys (drops the captured graph).
+static int batch_ensure_keys(Batch* b, int64_t nkeys) {
+  if (nkeys <= b->max_keys) return 0;
+  if (nkeys > (int64_t)INT32_MAX - 1) return fail(-1, "batch: sort key space exceeds 2^31");
+  NZ_HIP(hipStreamSynchronize(b->stream));
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  for (void* p : b->key_allocations) (void)hipFree(p);
+  b->key_allocations.clear();
+  const int64_t m = std::max<int64_t>(nkeys + nkeys / 2, 1 << 16);
+  void* p;
+  if (int rc = batch_alloc(b, sizeof(int32_t) * (m + 1), &p, b->key_allocations)) return rc;
+  b->counts = (int32_t*)p;
+  if (int rc = batch_alloc(b, sizeof(int32_t) * (m + 1), &p, b->key_allocations)) return rc;
+  b->cursor = (int32_t*)p;
+  if (int rc = batch_alloc(b, sizeof(int32_t) * ((m + 1 + kScanTile - 1) / kScanTile), &p,
+                           b->key_allocations))
+    return rc;
+  b->partial = (int32_t*)p;
+  b->max_keys = m;
+  return 0;
+}
+
+// Bring the device's progress fields to the host mirror (before the host
+// rewrites the tables) and upload the tables when they changed.
+static int batch_sync_tables(Batch* b, bool upload) {
+  NZ_HIP(hipStreamSynchronize(b->stream));
+  std::vector<SimDesc> dev(b->max_sims);
+  NZ_HIP(hipMemcpy(dev.data(), b->descs_dev, sizeof(SimDesc) * b->max_sims, hipMemcpyDeviceToHost));
+  for (int32_t s = 0; s < b->max_sims; ++s) {
+    b->descs[s].step = dev[s].step;
+    b->descs[s].active = dev[s].active;
+  }
+  if (!upload) return 0;
+  const int64_t nkeys = batch_layout(b);
+  if (int rc = batch_ensure_keys(b, nkeys)) return rc;
+  NZ_HIP(hipMemcpy(b->descs_dev, b->descs.data(), sizeof(SimDesc) * b->max_sims,
+                   hipMemcpyHostToDevice));
+  NZ_HIP(hipMemcpy(b->block_sim_dev, b->block_sim.data(), sizeof(int32_t) * b->nblocks,
+                   hipMemcpyHostToDevice));
+  NZ_HIP(hipMemcpy(b->block_key_dev, b->block_key.data(), sizeof(int64_t) * b->nblocks,
+                   hipMemcpyHostToDevice));
+  NZ_HIP(hipMemcpy(b->nkeys_dev, &nkeys, sizeof(int64_t), hipMemcpyHostToDevice));
+  b->dirty = false;
+  return 0;
+}
+
+// One launch: sort every simulation's spins, walk, advance.
+static int batch_enqueue_launch(Batch* b) {
+  hipStream_t s = b->stream;
+  const SlotState& src = b->buf[b->cur];
+  const SlotState& dst = b->buf[1 - b->cur];
+  const int64_t mk = b->max_keys;
+  hipLaunchKernelGGL(fill_i32_dev, dim3(blocks_for(mk + 1)), dim3(256), 0, s, b->counts,
+                     (const int64_t*)b->nkeys_dev, (int64_t)1, 0);
+  hipLaunchKernelGGL(fill_i32_dev, dim3(blocks_for(mk)), dim3(256), 0, s, b->cursor,
+                     (const int64_t*)b->nkeys_dev, (int64_t)0, 0);
+  hipLaunchKernelGGL(batch_keys_kernel, dim3(blocks_for(b->cap)), dim3(256), 0, s,
+                     (const SimDesc*)b->descs_dev, (const int32_t*)b->block_sim_dev,
+                     (const int64_t*)b->block_key_dev, src, b->cap, b->keys, b->counts);
+  const int64_t nbt = (mk + 1 + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(tile_sum_kernel, dim3(nbt), dim3(kScanThreads), 0, s, b->counts, mk + 1,
+                     b->partial, (const int64_t*)b->nkeys_dev);
+  hipLaunchKernelGGL(partial_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, b->partial, nbt,
+                     (const int64_t*)b->nkeys_dev);
+  hipLaunchKernelGGL(tile_scan_kernel, dim3(nbt), dim3(kScanThreads), 0, s, b->counts, mk + 1,
+                     (const int32_t*)b->partial, (const int64_t*)b->nkeys_dev);
+  hipLaunchKernelGGL(sort_scatter_kernel, dim3(blocks_for(b->cap)), dim3(256), 0, s, src, dst,
+                     b->cap, (const int32_t*)b->keys, (const int32_t*)b->counts, b->cursor);
+  hipLaunchKernelGGL(batch_walk_kernel, dim3((unsigned)b->nblocks), dim3(256), 0, s,
+                     (const SimDesc*)b->descs_dev, (const int32_t*)b->block_sim_dev, dst, b->cap,
+                     b->xcd_chunk);
+  hipLaunchKernelGGL(batch_advance_kernel, dim3(blocks_for(b->max_sims)), dim3(256), 0, s,
+                     b->descs_dev, b->max_sims);
+  NZ_HIP(hipGetLastError());
+  b->cur = 1 - b->cur;
+  return 0;
+}
+
+static int check_batch(const Batch* b, const char* who) {
+  if (!b) return fail(-1, std::string(who) + ": batch is NULL");
+  return 0;
+}
+
+static int check_sim(const Batch* b, int32_t sim, const char* who) {
+  if (int rc = check_batch(b, who)) return rc;
+  if (sim < 0 || sim >= b->max_sims || !b->jobs[sim].used)
+    return fail(-1, std::string(who) + ": no such simulation");
+  return 0;
+}
+
+// SlotState of a simulation's range (pointers offset to its first slot).
+static SlotState batch_view(const Batch* b, int32_t sim) {
+  SlotState v = b->buf[b->cur];
+  const int64_t o = b->jobs[sim].b0 * kBlk;
+  v.spin_id += o;
+  v.ucell += o;
+  v.rel += o;
+  v.ucell0 += o;
+  v.rel0 += o;
+  v.sid += o;
+  v.logsig += o;
+  v.mom += o;
+  v.tstep += o;
+  return v;
 }
 
 }  // namespace nz
@@ -719,7 +1066,7 @@ int nz_walker_read(nz_walker_t ww, float* pos, float* pos0, float* sig, float* m
   Walker* w = (Walker*)ww;
   if (int rc = check_walker(w, "nz_walker_read")) return rc;
   hipLaunchKernelGGL(read_kernel, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)stream,
-                     w->sub->g, w->buf[w->cur], w->n, pos, pos0, sig, mom, (double*)nullptr,
+                     w->sub->g, w->buf[w->cur], w->n, w->n, w->n, pos, pos0, sig, mom, (double*)nullptr,
                      (double*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
   NZ_HIP(hipGetLastError());
   return 0;
@@ -729,7 +1076,7 @@ int nz_walker_read_f64(nz_walker_t ww, double* pos, double* disp, void* stream) 
   Walker* w = (Walker*)ww;
   if (int rc = check_walker(w, "nz_walker_read_f64")) return rc;
   hipLaunchKernelGGL(read_kernel, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)stream,
-                     w->sub->g, w->buf[w->cur], w->n, (float*)nullptr, (float*)nullptr,
+                     w->sub->g, w->buf[w->cur], w->n, w->n, w->n, (float*)nullptr, (float*)nullptr,
                      (float*)nullptr, (float*)nullptr, pos, disp, (int32_t*)nullptr,
                      (int32_t*)nullptr);
   NZ_HIP(hipGetLastError());
@@ -742,7 +1089,7 @@ int nz_walker_counters(nz_walker_t ww, int32_t* nsub, int32_t* nguard, void* str
   if (!(w->flags & NZ_WALKER_COUNTERS))
     return fail(-1, "nz_walker_counters: walker created without NZ_WALKER_COUNTERS");
   hipLaunchKernelGGL(read_kernel, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)stream,
-                     w->sub->g, w->buf[w->cur], w->n, (float*)nullptr, (float*)nullptr,
+                     w->sub->g, w->buf[w->cur], w->n, w->n, w->n, (float*)nullptr, (float*)nullptr,
                      (float*)nullptr, (float*)nullptr, (double*)nullptr, (double*)nullptr, nsub,
                      nguard);
   NZ_HIP(hipGetLastError());
@@ -783,6 +1130,287 @@ int nz_displacement_moments(nz_walker_t ww, double* out, void* stream) {
                      w->sub->g, w->buf[w->cur], w->n, out);
   NZ_HIP(hipGetLastError());
   return 0;
+}
+
+
+int nz_batch_create(int64_t capacity, int32_t max_sims, nz_batch_t* out) {
+  if (!out) return fail(-1, "nz_batch_create: out is NULL");
+  *out = nullptr;
+  if (capacity <= 0 || capacity > (int64_t)INT32_MAX - kBlk)
+    return fail(-1, "nz_batch_create: capacity must be in [1, 2^31 - 256)");
+  if (max_sims <= 0 || max_sims > 65536) return fail(-1, "nz_batch_create: max_sims must be in [1, 65536]");
+  Batch* b = new Batch();
+  b->nblocks = (capacity + kBlk - 1) / kBlk;
+  b->cap = b->nblocks * kBlk;
+  b->max_sims = max_sims;
+  (void)hipGetDevice(&b->device);
+  if (const char* e = getenv("NZ_WALK_XCD_CHUNK")) b->xcd_chunk = std::max(0, atoi(e));
+  b->descs.assign(max_sims, SimDesc{});
+  b->jobs.assign(max_sims, BatchJob{});
+  b->block_sim.assign(b->nblocks, -1);
+  b->block_key.assign(b->nblocks, 0);
+  const int64_t n = b->cap;
+  int rc = 0;
+  void* p = nullptr;
+  auto A = [&](size_t bytes) { return batch_alloc(b, bytes, &p, b->allocations); };
+  for (int k = 0; k < 2 && !rc; ++k) {
+    SlotState& s = b->buf[k];
+    if ((rc = A(sizeof(int32_t) * n))) break;
+    s.spin_id = (int32_t*)p;
+    if ((rc = A(sizeof(int32_t) * 3 * n))) break;
+    s.ucell = (int32_t*)p;
+    if ((rc = A(sizeof(double) * 3 * n))) break;
+    s.rel = (double*)p;
+    if ((rc = A(sizeof(int32_t) * 3 * n))) break;
+    s.ucell0 = (int32_t*)p;
+    if ((rc = A(sizeof(double) * 3 * n))) break;
+    s.rel0 = (double*)p;
+    if ((rc = A(sizeof(int32_t) * n))) break;
+    s.sid = (int32_t*)p;
+    if ((rc = A(sizeof(float) * n))) break;
+    s.logsig = (float*)p;
+    if ((rc = A(sizeof(float) * 3 * n))) break;
+    s.mom = (float*)p;
+    if ((rc = A(sizeof(uint32_t) * n))) break;
+    s.tstep = (uint32_t*)p;
+    s.nsub = s.nguard = nullptr;
+  }
+  if (!rc && !(rc = A(sizeof(int32_t) * n))) b->keys = (int32_t*)p;
+  if (!rc && !(rc = A(sizeof(float) * 3 * n))) b->pos_tmp = (float*)p;
+  if (!rc && !(rc = A(sizeof(int64_t)))) b->nkeys_dev = (int64_t*)p;
+  if (!rc && !(rc = A(sizeof(int32_t)))) b->nfailed_dev = (int32_t*)p;
+  if (!rc && !(rc = A(sizeof(SimDesc) * max_sims))) b->descs_dev = (SimDesc*)p;
+  if (!rc && !(rc = A(sizeof(int32_t) * b->nblocks))) b->block_sim_dev = (int32_t*)p;
+  if (!rc && !(rc = A(sizeof(int64_t) * b->nblocks))) b->block_key_dev = (int64_t*)p;
+  if (!rc) {
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
+    if (e != hipSuccess) rc = hip_fail(e, "nz_batch_create: stream/events");
+  }
+  // every slot starts as free padding
+  for (int k = 0; k < 2 && !rc; ++k) {
+    hipLaunchKernelGGL(fill_i32, dim3(blocks_for(n)), dim3(256), 0, b->stream, b->buf[k].spin_id, n,
+                       -1);
+    hipLaunchKernelGGL(fill_u32, dim3(blocks_for(n)), dim3(256), 0, b->stream, b->buf[k].tstep, n, 0u);
+  }
+  if (!rc) {
+    const hipError_t e = hipMemcpy(b->descs_dev, b->descs.data(), sizeof(SimDesc) * max_sims,
+                                   hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_fail(e, "nz_batch_create: descriptors");
+  }
+  if (!rc) rc = batch_sync_tables(b, true);
+  if (rc) {
+    batch_release(b);
+    delete b;
+    return rc;
+  }
+  *out = (nz_batch_t)b;
+  return 0;
+}
+
+int nz_batch_destroy(nz_batch_t bb) {
+  if (!bb) return 0;
+  Batch* b = (Batch*)bb;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_release(b);
+  (void)hipSetDevice(dev);
+  delete b;
+  return 0;
+}
+
+int nz_batch_add(nz_batch_t bb, nz_substrate_t sub, int64_t n, uint64_t seed, int64_t nsteps,
+                 double dt, const float* g, const float* garea, uint32_t flags, int32_t k,
+                 const float* pos, const int32_t* targets_host, int32_t ntargets, int32_t* sim_out,
+                 void* stream) {
+  Batch* b = (Batch*)bb;
+  if (int rc = check_batch(b, "nz_batch_add")) return rc;
+  if (!sub || !sim_out) return fail(-1, "nz_batch_add: NULL substrate or sim_out");
+  if (n <= 0 || nsteps < 0 || nsteps > (int64_t)UINT32_MAX || !(dt >= 0.0) || k <= 0)
+    return fail(-1, "nz_batch_add: need n > 0, 0 <= nsteps < 2^32, dt >= 0, k > 0");
+  if ((g == nullptr) != (garea == nullptr)) return fail(-1, "nz_batch_add: pass both g and garea or neither");
+  if (!pos && (!targets_host || ntargets <= 0)) return fail(-1, "nz_batch_add: need pos or targets");
+  const Substrate* S = (const Substrate*)sub;
+  const int64_t nb = (n + kBlk - 1) / kBlk;
+  int32_t sim = -1;
+  for (int32_t s = 0; s < b->max_sims; ++s)
+    if (!b->jobs[s].used) { sim = s; break; }
+  int64_t b0 = -1;
+  for (int64_t pb = 0, run = 0; pb < b->nblocks; ++pb) {  // first fit
+    run = (b->block_sim[pb] < 0) ? run + 1 : 0;
+    if (run == nb) { b0 = pb - nb + 1; break; }
+  }
+  if (sim < 0 || b0 < 0) return fail(NZ_EFULL, "nz_batch_add: no room (simulation slots or contiguous spin blocks)");
+  if (int rc = batch_sync_tables(b, false)) return rc;
+  // order the caller's stream work (pos, g) before ours
+  NZ_HIP(hipEventRecord(b->ev_in, (hipStream_t)stream));
+  NZ_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+  const float* p0 = pos;
+  if (!p0) {
+    hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, b->stream, b->nfailed_dev, (int64_t)1, 0);
+    if (int rc = nz_seed(sub, n, targets_host, ntargets, seed ^ 0x5EED5EED5EEDull, 4096,
+                         b->pos_tmp, b->nfailed_dev, b->stream))
+      return rc;
+    int32_t nf = 0;
+    NZ_HIP(hipMemcpyAsync(&nf, b->nfailed_dev, sizeof(nf), hipMemcpyDeviceToHost, b->stream));
+    NZ_HIP(hipStreamSynchronize(b->stream));
+    if (nf) return fail(-1, "nz_batch_add: " + std::to_string(nf) + " spins could not be seeded "
+                            "(target structures empty or too small)");
+    p0 = b->pos_tmp;
+  }
+  const int64_t slot0 = b0 * kBlk;
+  for (int k2 = 0; k2 < 2; ++k2)  // both buffers: the range holds this simulation whichever is current
+    hipLaunchKernelGGL(batch_init_kernel, dim3(blocks_for(nb * kBlk)), dim3(256), 0, b->stream,
+                       S->g, b->buf[k2], b->cap, slot0, nb * kBlk, n, p0);
+  NZ_HIP(hipGetLastError());
+  SimDesc& d = b->descs[sim];
+  d = SimDesc{};
+  d.g = S->g;
+  d.seed = seed;
+  d.dt = dt;
+  d.gph = g;
+  d.garea = garea;
+  d.flags = flags;
+  d.active = nsteps > 0 ? 1 : 0;
+  d.nsteps = nsteps;
+  d.step = 0;
+  d.k = k;
+  d.ncells = (int64_t)S->g.ncell[0] * S->g.ncell[1] * S->g.ncell[2];
+  b->jobs[sim] = BatchJob{true, S, n, b0, nb};
+  for (int64_t pb = b0; pb < b0 + nb; ++pb) b->block_sim[pb] = sim;
+  // the progress fields of the new entry must not be overwritten by the
+  // device's stale copy: upload now
+  const int64_t nkeys = batch_layout(b);
+  if (int rc = batch_ensure_keys(b, nkeys)) return rc;
+  NZ_HIP(hipStreamSynchronize(b->stream));
+  NZ_HIP(hipMemcpy(b->descs_dev, b->descs.data(), sizeof(SimDesc) * b->max_sims, hipMemcpyHostToDevice));
+  NZ_HIP(hipMemcpy(b->block_sim_dev, b->block_sim.data(), sizeof(int32_t) * b->nblocks,
+                   hipMemcpyHostToDevice));
+  NZ_HIP(hipMemcpy(b->block_key_dev, b->block_key.data(), sizeof(int64_t) * b->nblocks,
+                   hipMemcpyHostToDevice));
+  NZ_HIP(hipMemcpy(b->nkeys_dev, &nkeys, sizeof(int64_t), hipMemcpyHostToDevice));
+  b->dirty = false;
+  *sim_out = sim;
+  return 0;
+}
+
+int nz_batch_remove(nz_batch_t bb, int32_t sim) {
+  Batch* b = (Batch*)bb;
+  if (int rc = check_sim(b, sim, "nz_batch_remove")) return rc;
+  if (int rc = batch_sync_tables(b, false)) return rc;
+  const BatchJob j = b->jobs[sim];
+  for (int64_t pb = j.b0; pb < j.b0 + j.nb; ++pb) b->block_sim[pb] = -1;
+  b->jobs[sim] = BatchJob{};
+  b->descs[sim] = SimDesc{};
+  // its slots become free padding in both buffers
+  for (int k2 = 0; k2 < 2; ++k2)
+    hipLaunchKernelGGL(fill_i32, dim3(blocks_for(j.nb * kBlk)), dim3(256), 0, b->stream,
+                       b->buf[k2].spin_id + j.b0 * kBlk, j.nb * kBlk, -1);
+  NZ_HIP(hipGetLastError());
+  return batch_sync_tables(b, true);
+}
+
+int nz_batch_run(nz_batch_t bb, int32_t launches, void* stream) {
+  Batch* b = (Batch*)bb;
+  if (int rc = check_batch(b, "nz_batch_run")) return rc;
+  if (launches < 0) return fail(-1, "nz_batch_run: launches < 0");
+  if (b->dirty)
+    if (int rc = batch_sync_tables(b, true)) return rc;
+  NZ_HIP(hipEventRecord(b->ev_in, (hipStream_t)stream));
+  NZ_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+  int32_t left = launches;
+  if (b->use_graph && left >= 2) {
+    if (b->graph && b->graph_cur != b->cur) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
+    if (!b->graph) {  // two launches: the slot buffers are back where they started
+      b->graph_cur = b->cur;
+      const int cur0 = b->cur;
+      NZ_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = batch_enqueue_launch(b);
+      if (!rc) rc = batch_enqueue_launch(b);
+      hipGraph_t gr = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->stream, &gr);
+      b->cur = cur0;
+      if (rc) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+      }
+      NZ_HIP(e);
+      const hipError_t e2 = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      NZ_HIP(e2);
+    }
+    for (; left >= 2; left -= 2) NZ_HIP(hipGraphLaunch(b->graph, b->stream));
+  }
+  for (; left > 0; --left)
+    if (int rc = batch_enqueue_launch(b)) return rc;
+  NZ_HIP(hipEventRecord(b->ev_out, b->stream));
+  NZ_HIP(hipStreamWaitEvent((hipStream_t)stream, b->ev_out, 0));
+  return 0;
+}
+
+int nz_batch_set_graph(nz_batch_t bb, int32_t enable) {
+  Batch* b = (Batch*)bb;
+  if (int rc = check_batch(b, "nz_batch_set_graph")) return rc;
+  b->use_graph = enable != 0;
+  return 0;
+}
+
+int nz_batch_status(nz_batch_t bb, int32_t sim, int64_t* step, int32_t* done) {
+  Batch* b = (Batch*)bb;
+  if (int rc = check_sim(b, sim, "nz_batch_status")) return rc;
+  if (int rc = batch_sync_tables(b, false)) return rc;
+  if (step) *step = b->descs[sim].step;
+  if (done) *done = (b->descs[sim].active == 0) ? 1 : 0;
+  return 0;
+}
+
+int nz_batch_free(nz_batch_t bb, int64_t* largest_run, int64_t* free_slots, int32_t* free_sims) {
+  Batch* b = (Batch*)bb;
+  if (int rc = check_batch(b, "nz_batch_free")) return rc;
+  int64_t best = 0, run = 0, total = 0;
+  for (int64_t pb = 0; pb < b->nblocks; ++pb) {
+    run = (b->block_sim[pb] < 0) ? run + 1 : 0;
+    total += (b->block_sim[pb] < 0);
+    best = std::max(best, run);
+  }
+  int32_t fs = 0;
+  for (const BatchJob& j : b->jobs) fs += !j.used;
+  if (largest_run) *largest_run = best * kBlk;
+  if (free_slots) *free_slots = total * kBlk;
+  if (free_sims) *free_sims = fs;
+  return 0;
+}
+
+int nz_batch_read(nz_batch_t bb, int32_t sim, float* pos, float* pos0, float* sig, float* mom,
+                  void* stream) {
+  Batch* b = (Batch*)bb;
+  if (int rc = check_sim(b, sim, "nz_batch_read")) return rc;
+  const BatchJob& j = b->jobs[sim];
+  NZ_HIP(hipEventRecord(b->ev_out, b->stream));
+  NZ_HIP(hipStreamWaitEvent((hipStream_t)stream, b->ev_out, 0));
+  hipLaunchKernelGGL(read_kernel, dim3(blocks_for(j.nb * kBlk)), dim3(256), 0, (hipStream_t)stream,
+                     j.sub->g, batch_view(b, sim), b->cap, j.nb * kBlk, j.n, pos, pos0, sig, mom,
+                     (double*)nullptr, (double*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
+  NZ_HIP(hipGetLastError());
+  return 0;
+}
+
+int nz_batch_signals(nz_batch_t bb, int32_t sim, int32_t W, const float* dirs, const float* scale,
+                     double* out, void* stream) {
+  Batch* b = (Batch*)bb;
+  if (int rc = check_sim(b, sim, "nz_batch_signals")) return rc;
+  const BatchJob& j = b->jobs[sim];
+  NZ_HIP(hipEventRecord(b->ev_out, b->stream));
+  NZ_HIP(hipStreamWaitEvent((hipStream_t)stream, b->ev_out, 0));
+  const SlotState v = batch_view(b, sim);
+  return signal_reduce(j.nb * kBlk, b->cap, v.spin_id, W, v.mom, dirs, scale, out,
+                       (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -40,3 +40,12 @@ def test_argument_errors_are_reported():
     w = ctypes.c_void_p()
     rc = L.nz_walker_create(None, 10, 0, ctypes.byref(w))
     assert rc < 0 and b"substrate" in L.nz_last_error_string()
+    b = ctypes.c_void_p()
+    rc = L.nz_batch_create(0, 4, ctypes.byref(b))
+    assert rc < 0 and b"capacity" in L.nz_last_error_string()
+    rc = L.nz_batch_run(None, 2, None)
+    assert rc < 0 and b"batch" in L.nz_last_error_string()
+    sim = ctypes.c_int32()
+    rc = L.nz_batch_add(None, None, 10, 0, 10, 0.01, None, None, 0, 64, None, None, 0,
+                        ctypes.byref(sim), None)
+    assert rc < 0 and b"batch" in L.nz_last_error_string()
