@@ -45,6 +45,11 @@ def parse():
                     help="spins of the CPU-baseline subsample (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=5000)
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--forms", type=int, default=1,
+                    help="also measure the batched and served forms (scaled; 0 = skip)")
+    ap.add_argument("--forms-steps", type=float, default=0.1,
+                    help="step-count scale of the batched/served workloads")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy a buffer of this many MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -183,7 +188,67 @@ def main():
     }
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(geom, prot, start, args)
+    if args.forms:
+        del walker, sub
+        torch.cuda.empty_cache()
+        out["forms"] = forms(args)
     print(json.dumps(out), flush=True)
+
+
+def forms(args):
+    """Batched and served forms against back-to-back plain walker launches of
+    the same simulations (each result checked against its run alone)."""
+    from nozomi_amd.serve import (Server, fifo_latencies, latency_stats, run_batched, run_serial,
+                                  serve_trace)
+    from nozomi_amd.serve import SubstrateCache
+    from nozomi_amd.workloads import batching_mix, pregenerate, serving_mix, total_spin_steps
+    res = {}
+    dev = torch.device("cuda:0")
+    # batching: 256 independent simulations of uneven size in one pool; their
+    # substrates are generated and uploaded before timing (resident in HBM)
+    reqs = batching_mix(nsims=256, seed=args.seed, scale_steps=args.forms_steps)
+    pregenerate(reqs)
+    cache = SubstrateCache(dev, 1 << 40)
+    cache.prefetch(reqs)
+    cap = sum(-(-r.n // 256) * 256 for r in reqs)
+    run_serial(reqs[:4], cache=cache)  # warms the kernels
+    got, t_b = run_batched(reqs, capacity=cap, k=64, cache=cache)
+    alone, t_s = run_serial(reqs, cache=cache)
+    diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
+    ss = total_spin_steps(reqs)
+    res["batching"] = {
+        "sims": len(reqs), "spin_steps": ss, "step_scale": args.forms_steps,
+        "batched_sims_per_s": len(reqs) / t_b, "serial_sims_per_s": len(reqs) / t_s,
+        "speedup_vs_serial": t_s / t_b, "batched_spin_steps_per_s": ss / t_b,
+        "serial_spin_steps_per_s": ss / t_s, "max_signal_diff_vs_alone": diff,
+    }
+    del cache
+    # serving: Poisson arrivals at 50 req/s, continuous batching, HBM admission;
+    # device substrates are built when a request first needs one
+    reqs, arr = serving_mix(rate=50.0, duration=args.serve_seconds, seed=args.seed + 1,
+                            scale_steps=args.forms_steps)
+    pregenerate(reqs)
+    cap = 16 * 1024 * 1024
+    srv = Server(cap, hbm_budget=64 << 30, max_sims=256, k=64)
+    got, makespan = serve_trace(srv, reqs, arr)
+    built, reused = srv.subs.built, srv.subs.reused
+    srv.close()
+    alone, t_s = run_serial(reqs)
+    diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
+    ss = total_spin_steps(reqs)
+    lat_serial = fifo_latencies(arr, [alone[r.rid].latency for r in reqs])
+    res["serving"] = {
+        "requests": len(reqs), "rate_per_s": 50.0, "arrival_window_s": args.serve_seconds,
+        "step_scale": args.forms_steps, "offered_spin_steps_per_s": ss / args.serve_seconds,
+        "makespan_s": makespan, "served_sims_per_s": len(reqs) / makespan,
+        "served_spin_steps_per_s": ss / makespan, "serial_sims_per_s": len(reqs) / t_s,
+        "latency": latency_stats(got),
+        "serial_fifo_latency": {"p50_s": float(np.percentile(lat_serial, 50)),
+                                "p99_s": float(np.percentile(lat_serial, 99))},
+        "substrates_built": built, "substrates_reused": reused,
+        "max_signal_diff_vs_alone": diff,
+    }
+    return res
 
 
 def measured_traffic(n, steps_per_launch, info):

@@ -189,6 +189,9 @@ int nz_batch_run(nz_batch_t b, int32_t launches, void* stream);
 int nz_batch_set_graph(nz_batch_t b, int32_t enable);
 /* Steps completed and whether the simulation finished (waits for the batch). */
 int nz_batch_status(nz_batch_t b, int32_t sim, int64_t* step, int32_t* done);
+/* The same for every simulation entry at once (arrays of max_sims; unused
+ * entries report step -1, done -1). */
+int nz_batch_status_all(nz_batch_t b, int64_t* steps, int32_t* done);
 /* Free capacity: largest contiguous run of free slots, all free slots, free
  * simulation entries. */
 int nz_batch_free(nz_batch_t b, int64_t* largest_run, int64_t* free_slots, int32_t* free_sims);

@@ -78,6 +78,7 @@ _SIGS = {
     "nz_batch_set_graph": (ctypes.c_int, [_c_vp, ctypes.c_int32]),
     "nz_batch_status": (ctypes.c_int, [_c_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
                                        ctypes.POINTER(ctypes.c_int32)]),
+    "nz_batch_status_all": (ctypes.c_int, [_c_vp, _c_vp, _c_vp]),
     "nz_batch_free": (ctypes.c_int, [_c_vp, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
     "nz_batch_read": (ctypes.c_int, [_c_vp, ctypes.c_int32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),

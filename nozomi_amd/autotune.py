@@ -1,0 +1,47 @@
+"""Autotuner of the walk's launch parameters for a substrate.
+
+The walk kernel runs one spin per lane; what changes its speed on a given
+substrate is the cell edge of the candidate grid (candidates per list against
+re-bins), the list margin (list length against safety-box fallbacks) and the
+steps per launch, which is also the spatial re-sort interval (sort cost
+against wavefront coherence).  ``tune`` times a short walk of a subsample for
+every combination and returns the fastest; results are cached per key.
+"""
+import itertools
+import time
+
+import numpy as np
+import torch
+
+from . import runtime as rt
+
+_CACHE = {}
+
+
+def tune(geom, key=None, n=65536, steps=128, dt=0.01, cells=(1.25, 1.5, 2.0),
+         margins=(0.7, 0.8, 1.0), ks=(32, 64, 128), device=None, seed=1):
+    """Returns {"cell": c, "margin": m, "k": k, "ns_per_spin_step": t,
+    "table": [(t, c, m, k), ...]} for the geometry (a SimGeometry3D)."""
+    if key is not None and key in _CACHE:
+        return _CACHE[key]
+    device = rt._require_gpu(device)
+    table = []
+    for cell, margin in itertools.product(cells, margins):
+        sub = geom.device_substrate(cell_size=cell, margin=margin, device=device)
+        p = sub.seed(n, targets=np.arange(geom.nstructures + 1), seed=seed)
+        for k in ks:
+            w = rt.Walker(sub, n, sort_interval=k)
+            w.set_positions(p)
+            w.walk(seed, k, dt, t2=False)  # first sort and launch
+            torch.cuda.synchronize(device)
+            t0 = time.perf_counter()
+            w.walk(seed, steps, dt, t2=False)
+            torch.cuda.synchronize(device)
+            table.append(((time.perf_counter() - t0) / (n * steps) * 1e9, cell, margin, k))
+            w.close()
+        sub.close()
+    t, cell, margin, k = min(table)
+    out = {"cell": cell, "margin": margin, "k": k, "ns_per_spin_step": t, "table": table}
+    if key is not None:
+        _CACHE[key] = out
+    return out

@@ -106,6 +106,16 @@ class Batch:
                   "nz_batch_status")
         return step.value, bool(done.value)
 
+    def status_all(self):
+        """(steps [max_sims], done [max_sims]); unused entries hold -1."""
+        steps = np.zeros(self.max_sims, dtype=np.int64)
+        done = np.zeros(self.max_sims, dtype=np.int32)
+        with torch.cuda.device(self.device):
+            check(lib().nz_batch_status_all(self.handle, steps.ctypes.data_as(ctypes.c_void_p),
+                                            done.ctypes.data_as(ctypes.c_void_p)),
+                  "nz_batch_status_all")
+        return steps, done
+
     def free(self):
         """(largest contiguous free slots, free slots, free simulation entries)."""
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()

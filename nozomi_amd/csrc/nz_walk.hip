@@ -1370,6 +1370,18 @@ int nz_batch_status(nz_batch_t bb, int32_t sim, int64_t* step, int32_t* done) {
   return 0;
 }
 
+int nz_batch_status_all(nz_batch_t bb, int64_t* steps, int32_t* done) {
+  Batch* b = (Batch*)bb;
+  if (int rc = check_batch(b, "nz_batch_status_all")) return rc;
+  if (int rc = batch_sync_tables(b, false)) return rc;
+  for (int32_t s = 0; s < b->max_sims; ++s) {
+    const bool used = b->jobs[s].used;
+    if (steps) steps[s] = used ? b->descs[s].step : -1;
+    if (done) done[s] = used ? (b->descs[s].active == 0 ? 1 : 0) : -1;
+  }
+  return 0;
+}
+
 int nz_batch_free(nz_batch_t bb, int64_t* largest_run, int64_t* free_slots, int32_t* free_sims) {
   Batch* b = (Batch*)bb;
   if (int rc = check_batch(b, "nz_batch_free")) return rc;
