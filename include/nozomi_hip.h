@@ -202,6 +202,34 @@ int nz_batch_read(nz_batch_t b, int32_t sim, float* pos, float* pos0, float* sig
 int nz_batch_signals(nz_batch_t b, int32_t sim, int32_t W, const float* dirs, const float* scale,
                      double* out, void* stream);
 
+/* ---- precision form: blocked phase contraction of many waveforms ----------
+ * (no counterpart in the reference, whose phase kernels take one waveform
+ * shape per launch, sim3d_kernel.cu:719-943).  For W waveform shapes g_w(t)
+ * with directions u_w: phase[i,w] = (x0_i . u_w) sum_t g_w(t) (nz_phase_init)
+ * + sum over K-step blocks of rec . Bt (nz_phase_gemm), where rec holds the
+ * block's unwrapped displacements (nz_walk_record) and Bt[w][3(t-t0)+a] =
+ * g_w(t) u_w,a (nz_waveform_block). */
+/* Walk nsteps (<= 4096) with every spin stopping at the block end, recording
+ * FP32 displacements from the start: rec [3*nsteps][n] (row 3t+a, slot
+ * order); slot_spin [n] receives the slot -> spin map of the record. */
+int nz_walk_record(nz_walker_t w, uint64_t seed, int32_t nsteps, double dt, float* rec,
+                   int32_t* slot_spin, void* stream);
+/* Waveform block for steps t0..t0+K-1: g [W][T] per-step coefficients,
+ * u [3][W] directions; Bt as BF16 hi/lo [W][3K] (uint16 bits) and, if
+ * bt_f32 is not NULL, FP32. */
+int nz_waveform_block(const float* g, int64_t T, const float* u, int32_t W, int64_t t0, int32_t K,
+                      uint16_t* bt_hi, uint16_t* bt_lo, float* bt_f32, void* stream);
+/* phase [n_spins][W] += rec^T . Bt over the block: mode 0 = split hi/lo BF16
+ * MFMA (K a multiple of 32), mode 1 = FP32 vector ALUs. */
+int nz_phase_gemm(const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin,
+                  int32_t K, const uint16_t* bt_hi, const uint16_t* bt_lo, const float* bt_f32,
+                  int32_t W, int32_t mode, float* phase, void* stream);
+/* phase [n][W] = (x0_i . u_w) gsum[w]; x0 [3][n] start positions. */
+int nz_phase_init(const float* x0, int64_t n, const float* u, const float* gsum, int32_t W,
+                  float* phase, void* stream);
+/* out [2W] = sum_i cos / sin phase[i][w], FP64. */
+int nz_phase_signals(const float* phase, int64_t n, int32_t W, double* out, void* stream);
+
 /* Phases of W directions from the moments: phase[w, i] = scale[w] *
  * (dirs[0,w] m_x + dirs[1,w] m_y + dirs[2,w] m_z).  Replaces the per-step
  * W-wide accumulation of randomWalk3d_phase_multidirr (sim3d_kernel.cu:880-942);

@@ -67,6 +67,16 @@ _SIGS = {
     "nz_walker_counters": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp]),
     "nz_displacement_moments": (ctypes.c_int, [_c_vp, _c_vp, _c_vp]),
     "nz_walker_time_kernels": (ctypes.c_int, [_c_vp, ctypes.c_int32]),
+    "nz_walk_record": (ctypes.c_int, [_c_vp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double, _c_vp,
+                                      _c_vp, _c_vp]),
+    "nz_waveform_block": (ctypes.c_int, [_c_vp, ctypes.c_int64, _c_vp, ctypes.c_int32, ctypes.c_int64,
+                                         ctypes.c_int32, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "nz_phase_gemm": (ctypes.c_int, [_c_vp, ctypes.c_int64, ctypes.c_int64, _c_vp, ctypes.c_int32,
+                                     _c_vp, _c_vp, _c_vp, ctypes.c_int32, ctypes.c_int32, _c_vp,
+                                     _c_vp]),
+    "nz_phase_init": (ctypes.c_int, [_c_vp, ctypes.c_int64, _c_vp, _c_vp, ctypes.c_int32, _c_vp,
+                                     _c_vp]),
+    "nz_phase_signals": (ctypes.c_int, [_c_vp, ctypes.c_int64, ctypes.c_int32, _c_vp, _c_vp]),
     "nz_batch_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_c_vp)]),
     "nz_batch_destroy": (ctypes.c_int, [_c_vp]),
     "nz_batch_add": (ctypes.c_int, [_c_vp, _c_vp, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int64,

@@ -68,6 +68,7 @@ struct WalkArgs {
   double* stepmom;
   uint32_t flags;
   int32_t xcd_chunk;  // slot blocks per XCD chunk (0: no remap)
+  float* rec;         // displacement record (nz_walk_record) or null
 };
 
 // Workgroups of a launch are dispatched to the 8 XCDs round-robin (block b
@@ -127,6 +128,7 @@ struct StepParams {
   const float* garea;
   double* stepmom;   // per-step moment sums (needs smom and target - mom_base <= kMomChunk)
   uint32_t flags;
+  float* rec;        // per-step displacement record [3 * (target - mom_base)][ld] or null
 };
 
 // Walk the spin held in slot i (fields strided by ld) from its next step to
@@ -156,6 +158,8 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   if (P.stepmom) {
     for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x) smom[k] = 0.0f;
     __syncthreads();
+  }
+  if (P.stepmom || P.rec) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       u0[a] = active ? st.ucell0[a * ld + ii] : 0;
@@ -181,6 +185,11 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
         const float G = P.gph ? P.gph[t - P.base] : 0.0f;
         const float Ga = P.gph ? P.garea[t - P.base] : 0.0f;
         step_end(g, s, G, Ga);
+        if (P.rec) {  // displacement since the start, for the blocked phase contraction
+          const int64_t r = (t - P.mom_base) * 3;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) P.rec[(r + a) * ld + i] = (float)lane_disp(g, s, u0, p0, a);
+        }
         if (P.stepmom) {  // moment sums (sim3d_kernel.cu:191-237)
           const int k = (int)(t - P.mom_base) * 6;
 #pragma unroll
@@ -219,8 +228,8 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A
   __shared__ float smom[kMomChunk * 6];
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
   const int64_t i = blk * blockDim.x + threadIdx.x;
-  const StepParams P{A.seed, A.base, A.target, A.limit, A.mom_base, A.dt,
-                     A.gph,  A.garea, A.stepmom, A.flags};
+  const StepParams P{A.seed,  A.base,    A.target, A.limit, A.mom_base, A.dt,
+                     A.gph,   A.garea,   A.stepmom, A.flags, A.rec};
   walk_slot(A.g, A.st, A.n, i, i < A.n, P, smom);
 }
 
@@ -721,7 +730,8 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR batch_walk_kernel(const SimD
   if (!d.active) return;
   const int64_t i = pb * kBlk + threadIdx.x;
   const int64_t target = min(d.step + d.k, d.nsteps);
-  const StepParams P{d.seed, 0, target, d.nsteps, 0, d.dt, d.gph, d.garea, nullptr, d.flags};
+  const StepParams P{d.seed, 0, target, d.nsteps, 0, d.dt, d.gph, d.garea, nullptr, d.flags,
+                     nullptr};
   walk_slot(d.g, st, ld, i, st.spin_id[i] >= 0, P, nullptr);
 }
 
@@ -1047,6 +1057,7 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
     a.stepmom = stepmom;
     a.flags = flags;
     a.xcd_chunk = w->xcd_chunk;
+    a.rec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (w->timing) {
       if (int rc = next_event_pair(w, &e0, &e1)) return rc;
@@ -1059,6 +1070,42 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
     w->since_sort += (int32_t)chunk;
   }
   w->step = end;
+  return 0;
+}
+
+int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, float* rec,
+                   int32_t* slot_spin, void* stream) {
+  Walker* w = (Walker*)ww;
+  if (int rc = check_walker(w, "nz_walk_record")) return rc;
+  if (nsteps <= 0 || nsteps > 4096) return fail(-1, "nz_walk_record: nsteps must be in [1, 4096]");
+  if (!(dt >= 0.0)) return fail(-1, "nz_walk_record: dt must be >= 0");
+  if (!rec || !slot_spin) return fail(-1, "nz_walk_record: NULL rec or slot_spin");
+  if (w->step + nsteps > (int64_t)UINT32_MAX) return fail(-1, "nz_walk_record: step counter overflow");
+  hipStream_t s = (hipStream_t)stream;
+  if (w->sort_interval > 0 && w->since_sort >= w->sort_interval)
+    if (int rc = sort_spins(w, s)) return rc;
+  WalkArgs a;
+  a.g = w->sub->g;
+  a.st = w->buf[w->cur];
+  a.n = w->n;
+  a.seed = seed;
+  a.base = w->step;
+  a.mom_base = w->step;
+  a.target = w->step + nsteps;
+  a.limit = a.target;  // every spin stops at the block end: the record is complete
+  a.dt = dt;
+  a.gph = nullptr;
+  a.garea = nullptr;
+  a.stepmom = nullptr;
+  a.flags = 0;
+  a.xcd_chunk = w->xcd_chunk;
+  a.rec = rec;
+  hipLaunchKernelGGL(walk_kernel, dim3(blocks_for(w->n)), dim3(256), 0, s, a);
+  NZ_HIP(hipGetLastError());
+  NZ_HIP(hipMemcpyAsync(slot_spin, w->buf[w->cur].spin_id, sizeof(int32_t) * w->n,
+                        hipMemcpyDeviceToDevice, s));
+  w->step += nsteps;
+  w->since_sort += nsteps;
   return 0;
 }
 

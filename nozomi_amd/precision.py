@@ -1,0 +1,125 @@
+"""The precision form of the DWI walker: many waveform shapes at once.
+
+W gradient waveforms (PGSE and apodized-cosine OGSE, |G| <= 300 mT/m, seeded
+random directions) sampled on the walk's time grid are contracted with the
+spins' displacements in blocks of K steps: [N x 3K] x [3K x W] per block on
+the MFMA units as split hi/lo BF16 with FP32 accumulation (nz_phase_gemm mode
+0), or on the vector ALUs in FP32 (mode 1).  The phases [N, W] stay resident
+in HBM; the signals sum exp(i phase) over spins in FP64 at the end.
+
+Phase per step: phi += g_w(t) u_w . x(t) with g_w(t) = gamma G_w(t) dt,
+gamma = 2.67513e-4 rad / (ms um mT/m) (267.513 rad/ms/mT, waveforms.py:4).
+Positions enter as start position + unwrapped displacement, so no wrap
+correction is involved (the plain walker keeps the reference's correction).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import runtime as rt
+from ._lib import check, lib
+from .runtime import _ptr, _stream_ptr
+from .simulation_engine import waveforms as wf
+
+GAMMA = 2.67513e-4  # rad / (ms um (mT/m))
+
+
+def precision_waveforms(W, T, dt, seed=20261203, gmax=300.0):
+    """(g [W, T] float32 per-step coefficients, u [3, W] float32 unit
+    directions, kinds [W] str).  Half PGSE (delta, Delta drawn to fit the
+    echo time T dt), half apodized cosine OGSE at 25-150 Hz."""
+    rng = np.random.default_rng(seed)
+    te = T * dt
+    g = np.zeros((W, T), np.float64)
+    kinds = []
+    t = dt * np.arange(1, T + 1)
+    for w in range(W):
+        amp = rng.uniform(0.2, 1.0) * gmax
+        if w % 2 == 0:
+            delta = rng.uniform(0.1, 0.4) * te
+            Delta = rng.uniform(delta + dt, te - delta)
+            on1 = (t > te / 2 - Delta / 2 - delta / 2) & (t < te / 2 - Delta / 2 + delta / 2)
+            on2 = (t > te / 2 + Delta / 2 - delta / 2) & (t < te / 2 + Delta / 2 + delta / 2)
+            wave = on1.astype(float) - on2.astype(float)
+            kinds.append("PGSE")
+        else:
+            f = rng.uniform(25.0, 150.0) / 1000.0  # cycles per ms
+            n_cyc = max(1, int(0.45 * te * f - 0.25))
+            dur = min((0.25 + n_cyc) / f, 0.48 * te)
+            o = wf.ApodizedCosineOGSEWaveform(n_cyc, dur, te=te, time_step=dt)
+            wave = np.resize(o.wave, T)
+            kinds.append("OGSE")
+        g[w] = GAMMA * amp * wave * dt
+    u = rng.normal(size=(3, W))
+    u /= np.linalg.norm(u, axis=0, keepdims=True)
+    return g.astype(np.float32), u.astype(np.float32), kinds
+
+
+class PrecisionRun:
+    """Walk N spins (resident walker) for T steps and contract every K-step
+    block with the W waveforms.  mode: "mfma" (split BF16) or "fp32"."""
+
+    def __init__(self, sub, x0, seed, g, u, dt, K=64, mode="mfma"):
+        self.device = sub.device
+        self.N = x0.shape[1]
+        self.W, self.T = g.shape
+        self.K = int(K)
+        if mode == "mfma" and self.K % 32:
+            raise ValueError("the MFMA contraction needs K a multiple of 32")
+        self.mode = 0 if mode == "mfma" else 1
+        self.seed, self.dt = int(seed), float(dt)
+        d = self.device
+        self.x0 = x0.to(d, torch.float32).contiguous()
+        self.g = torch.from_numpy(np.ascontiguousarray(g, np.float32)).to(d)
+        self.u = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(d)
+        self.gsum = torch.from_numpy(g.astype(np.float64).sum(1).astype(np.float32)).to(d)
+        self.walker = rt.Walker(sub, self.N, sort_interval=self.K)
+        self.phase = torch.empty((self.N, self.W), dtype=torch.float32, device=d)
+        self.rec = torch.zeros((3 * self.K, self.N), dtype=torch.float32, device=d)
+        self.slot_spin = torch.empty(self.N, dtype=torch.int32, device=d)
+        self.bt_hi = torch.empty((self.W, 3 * self.K), dtype=torch.int16, device=d)
+        self.bt_lo = torch.empty_like(self.bt_hi)
+        self.bt_f32 = torch.empty((self.W, 3 * self.K), dtype=torch.float32, device=d) \
+            if self.mode == 1 else None
+        self.signals_out = torch.empty((self.W, 2), dtype=torch.float64, device=d)
+
+    def reset(self):
+        L = lib()
+        with torch.cuda.device(self.device):
+            self.walker.set_positions(self.x0)
+            check(L.nz_phase_init(_ptr(self.x0), self.N, _ptr(self.u), _ptr(self.gsum), self.W,
+                                  _ptr(self.phase), _stream_ptr(self.device)), "nz_phase_init")
+
+    def block(self, t0, record_hook=None):
+        """Walk and contract steps t0 .. t0 + K - 1 (fewer at the end)."""
+        L = lib()
+        k = min(self.K, self.T - t0)
+        s = _stream_ptr(self.device)
+        with torch.cuda.device(self.device):
+            check(L.nz_walk_record(self.walker.handle, self.seed & (2**64 - 1), k, self.dt,
+                                   _ptr(self.rec), _ptr(self.slot_spin), s), "nz_walk_record")
+            if record_hook is not None:
+                record_hook(t0, k, self.rec, self.slot_spin)
+            check(L.nz_waveform_block(_ptr(self.g), self.T, _ptr(self.u), self.W, t0, self.K,
+                                      _ptr(self.bt_hi), _ptr(self.bt_lo), _ptr(self.bt_f32), s),
+                  "nz_waveform_block")
+            check(L.nz_phase_gemm(_ptr(self.rec), self.N, self.N, _ptr(self.slot_spin), self.K,
+                                  _ptr(self.bt_hi), _ptr(self.bt_lo), _ptr(self.bt_f32), self.W,
+                                  self.mode, _ptr(self.phase), s), "nz_phase_gemm")
+
+    def run(self, record_hook=None):
+        self.reset()
+        for t0 in range(0, self.T, self.K):
+            self.block(t0, record_hook)
+
+    def signals(self):
+        """Complex [W] = sum exp(i phase) / N (FP64 accumulation)."""
+        with torch.cuda.device(self.device):
+            check(lib().nz_phase_signals(_ptr(self.phase), self.N, self.W, _ptr(self.signals_out),
+                                         _stream_ptr(self.device)), "nz_phase_signals")
+        s = self.signals_out.cpu().numpy()
+        return (s[:, 0] + 1j * s[:, 1]) / self.N
+
+    def close(self):
+        self.walker.close()

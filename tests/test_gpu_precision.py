@@ -1,0 +1,94 @@
+"""The precision form: K-step blocks of displacements contracted with many
+waveforms as split hi/lo BF16 MFMA products (FP32 accumulation), against an
+FP64 contraction of the same blocks, the FP32 vector-ALU contraction, and the
+FP64 oracle's own walk of the same spins."""
+import numpy as np
+import pytest
+import torch
+
+from nozomi_amd._lib import check, lib
+from nozomi_amd.precision import PrecisionRun, precision_waveforms
+from nozomi_amd.runtime import _ptr, _stream_ptr
+from nozomi_amd.substrate import generate_beaded_axons, load_geometry
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def test_gemm_modes_against_fp64():
+    rng = np.random.default_rng(0)
+    K, N, W = 64, 1000, 200
+    rec = rng.normal(scale=5.0, size=(3 * K, N)).astype(np.float32)
+    bt = rng.normal(scale=1e-3, size=(W, 3 * K)).astype(np.float32)
+    slot_spin = rng.permutation(N).astype(np.int32)
+    slot_spin[::7] = -1                      # padding slots contribute nothing
+    ph0 = rng.normal(size=(N, W)).astype(np.float32)
+    ref = ph0.astype(np.float64).copy()
+    valid = slot_spin >= 0
+    ref[slot_spin[valid]] += rec[:, valid].T.astype(np.float64) @ bt.T.astype(np.float64)
+    # BF16 hi/lo of bt the way the library splits (round to nearest even)
+    u = bt.view(np.uint32).astype(np.uint64)
+    hi = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    lo_f = bt - (hi.astype(np.uint32) << 16).view(np.float32)
+    ul = lo_f.view(np.uint32).astype(np.uint64)
+    lo = ((ul + 0x7FFF + ((ul >> 16) & 1)) >> 16).astype(np.uint16)
+    t = {k: torch.from_numpy(v).to(DEV) for k, v in
+         dict(rec=rec, ss=slot_spin, hi=hi.view(np.int16), lo=lo.view(np.int16), f32=bt).items()}
+    scale = np.abs(rec).T.astype(np.float64) @ np.abs(bt).T.astype(np.float64)
+    for mode, tol in ((0, 6e-5), (1, 1e-5)):
+        ph = torch.from_numpy(ph0.copy()).to(DEV)
+        check(lib().nz_phase_gemm(_ptr(t["rec"]), N, N, _ptr(t["ss"]), K, _ptr(t["hi"]), _ptr(t["lo"]),
+                                  _ptr(t["f32"]), W, mode, _ptr(ph), _stream_ptr(DEV)))
+        got = ph.cpu().numpy().astype(np.float64)
+        err = np.abs(got - ref)
+        rel = err[slot_spin[valid]] / (scale[valid] + 1e-30)
+        print(f"mode {mode}: max error relative to sum |terms| = {rel.max():.2e}")
+        assert rel.max() < tol
+        untouched = np.setdiff1d(np.arange(N), slot_spin[valid])
+        assert np.array_equal(got[untouched], ph0[untouched].astype(np.float64))
+
+
+def test_precision_walk_against_oracle(oracle_lib):
+    M, L = generate_beaded_axons(16, 30.0, seed=20261203)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    sub = geom.device_substrate(device=DEV)
+    N, T, dt, W, K, seed = 4096, 256, 0.02, 48, 64, 4321
+    g, u, kinds = precision_waveforms(W, T, dt, seed=7)
+    x0 = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=11)
+    blocks = []
+
+    def hook(t0, k, rec, slot_spin):
+        blocks.append((t0, k, rec[:3 * k].cpu().numpy().copy(), slot_spin.cpu().numpy().copy()))
+
+    runs = {}
+    for mode in ("mfma", "fp32"):
+        pr = PrecisionRun(sub, x0, seed, g, u, dt, K=K, mode=mode)
+        pr.run(record_hook=hook if mode == "mfma" else None)
+        runs[mode] = (pr.phase.cpu().numpy().astype(np.float64), pr.signals())
+        pr.close()
+    # FP64 contraction of the recorded FP32 displacements: contraction error alone
+    x0h = x0.cpu().numpy().astype(np.float64)
+    g64, u64 = g.astype(np.float64), u.astype(np.float64)
+    ph_rec = (x0h.T @ u64) * g64.sum(1)[None, :]
+    for t0, k, rec, ss in blocks:
+        d = rec.reshape(k, 3, N)                                   # [t, a, slot]
+        contrib = np.einsum("tas,wt,aw->sw", d, g64[:, t0:t0 + k], u64)
+        ph_rec[ss] += contrib
+    for mode in ("mfma", "fp32"):
+        e = np.abs(runs[mode][0] - ph_rec).max()
+        print(f"{mode}: max |phase - FP64 contraction of the record| = {e:.2e} rad")
+        assert e < (2e-4 if mode == "mfma" else 2e-5)
+    # the FP64 oracle walking the same spins: phases from its own trajectories
+    orc = oracle_lib.OracleSubstrate(geom.spheres, geom.nstructures, geom.structure_D(),
+                                     geom.structure_T2(), (L, L, L))
+    pos, pos0 = x0h.copy(), x0h.copy()
+    ph_o = ph_rec * 0 + (x0h.T @ u64) * g64.sum(1)[None, :]
+    for t in range(T):
+        orc.walk(pos, pos0, np.ones(N), seed=seed, step0=t, nsteps=1, dt=dt)
+        ph_o += ((pos - pos0).T @ (u64 * g64[:, t][None, :]))
+    s_o = np.exp(1j * ph_o).mean(0)
+    for mode in ("mfma", "fp32"):
+        es = np.abs(runs[mode][1] - s_o).max()
+        print(f"{mode}: max |signal - FP64 oracle| = {es:.2e}")
+        assert es < 1e-3
+    assert np.abs(runs["mfma"][1] - runs["fp32"][1]).max() < 1e-4
