@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--forms-steps", type=float, default=0.1,
                     help="step-count scale of the batched/served workloads")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--precision-spins", type=int, default=4_000_000)
+    ap.add_argument("--precision-steps", type=int, default=2000,
+                    help="steps of the precision-form measurement (1e4 = the full workload)")
+    ap.add_argument("--precision-waveforms", type=int, default=1024)
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy a buffer of this many MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -189,15 +193,72 @@ def main():
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(geom, prot, start, args)
     if args.forms:
+        out["forms"] = {"precision": precision_form(args, geom, sub)}
         del walker, sub
         torch.cuda.empty_cache()
-        out["forms"] = forms(args)
+        out["forms"].update(forms(args))
     print(json.dumps(out), flush=True)
+
+
+def precision_form(args, geom, sub):
+    """W waveform shapes contracted with K = 64-step blocks of displacements:
+    split hi/lo BF16 MFMA against the FP32 vector-ALU contraction."""
+    import torch
+    from nozomi_amd.precision import PrecisionRun, precision_waveforms
+    N, T, W, K, dt = args.precision_spins, args.precision_steps, args.precision_waveforms, 64, 0.01
+    g, u, kinds = precision_waveforms(W, T, dt, seed=args.seed)
+    x0 = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=args.seed + 3)
+    res = {"spins": N, "steps": T, "waveforms": W, "K": K, "dt_ms": dt,
+           "workload": "200-axon substrate, 4e6 spins, 1024 PGSE/OGSE waveforms (25-150 Hz, "
+                       "|G| <= 300 mT/m, seeded directions), steps as given"}
+    sig = {}
+    for mode in ("mfma", "fp32"):
+        pr = PrecisionRun(sub, x0, args.seed, g, u, dt, K=K, mode=mode)
+        pr.reset()
+        pr.block(0)  # warm-up block (kernels, sort)
+        pr.reset()
+        torch.cuda.synchronize()
+        ev = []
+        stream = torch.cuda.current_stream()
+        t0 = time.perf_counter()
+        for b0 in range(0, T, K):
+            # time the contraction alone: events around nz_phase_gemm via the hook
+            holder = {}
+
+            def hook(_t0, _k, _rec, _ss, holder=holder):
+                e = torch.cuda.Event(enable_timing=True)
+                e.record(stream)
+                holder["a"] = e
+            pr.block(b0, record_hook=hook)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record(stream)
+            ev.append((holder["a"], e1))
+        sig[mode] = pr.signals()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        gemm_ms = [a.elapsed_time(b) for a, b in ev]
+        useful = 6.0 * W * N * K  # FLOP per full block
+        full = [m for m, (b0) in zip(gemm_ms, range(0, T, K)) if b0 + K <= T]
+        avg = float(np.mean(full))
+        res[mode] = {"seconds": el, "spin_steps_per_s": N * T / el,
+                     "contraction_ms_per_block": avg,
+                     "contraction_useful_tflops": useful / (avg * 1e-3) / 1e12}
+        pr.close()
+        del pr
+        torch.cuda.empty_cache()
+    peak = 2500.0  # dense BF16 MFMA TFLOP/s
+    m = res["mfma"]
+    m["contraction_executed_tflops"] = 3 * m["contraction_useful_tflops"]
+    m["roofline"] = {"bound": "mfma", "achieved": m["contraction_executed_tflops"], "peak": peak,
+                     "unit": "TFLOP/s", "frac": m["contraction_executed_tflops"] / peak}
+    res["max_signal_diff_mfma_vs_fp32"] = float(np.abs(sig["mfma"] - sig["fp32"]).max())
+    return res
 
 
 def forms(args):
     """Batched and served forms against back-to-back plain walker launches of
     the same simulations (each result checked against its run alone)."""
+    import torch
     from nozomi_amd.serve import (Server, fifo_latencies, latency_stats, run_batched, run_serial,
                                   serve_trace)
     from nozomi_amd.serve import SubstrateCache
