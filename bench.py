@@ -201,11 +201,11 @@ def main():
 
 
 def precision_form(args, geom, sub):
-    """W waveform shapes contracted with K = 64-step blocks of displacements:
+    """W waveform shapes contracted with K = 128-step blocks of displacements:
     split hi/lo BF16 MFMA against the FP32 vector-ALU contraction."""
     import torch
     from nozomi_amd.precision import PrecisionRun, precision_waveforms
-    N, T, W, K, dt = args.precision_spins, args.precision_steps, args.precision_waveforms, 64, 0.01
+    N, T, W, K, dt = args.precision_spins, args.precision_steps, args.precision_waveforms, 128, 0.01
     g, u, kinds = precision_waveforms(W, T, dt, seed=args.seed)
     x0 = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=args.seed + 3)
     res = {"spins": N, "steps": T, "waveforms": W, "K": K, "dt_ms": dt,

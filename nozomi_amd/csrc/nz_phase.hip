@@ -64,75 +64,118 @@ constexpr int kTileW = 64;   // waveforms per workgroup tile
 constexpr int kChunk = 32;   // k per MFMA (16x16x32 BF16)
 
 // phase[spin(s), w] += sum_k rec[k][s] * Bt[w][k] over one block of K3 rows,
-// 16x16x32 BF16 MFMAs, split hi/lo (three products).  Workgroup: 4 waves,
-// a 64-slot x 64-waveform tile; wave q owns slots [16q, 16q + 16).
+// 16x16x32 BF16 MFMAs, split hi/lo (three products).  Workgroup: 4 waves and
+// 64 slots; wave q owns slots [16q, 16q + 16) and walks every waveform in
+// tiles of 64.  The block's displacements are read from HBM once: staged
+// through LDS (coalesced rows -> per-lane k-contiguous vectors), split, and
+// kept in registers as the A operands of all waveform tiles; the B operands
+// (the waveform block, a few hundred KB shared by all workgroups) come from L2.
 // MFMA operand layout (lane l of 64): A row l % 16, k 8 (l / 16) .. + 8;
 // B column l % 16, the same k; D column l % 16, rows 4 (l / 16) .. + 4.
+constexpr int kStageK3 = 192;  // rows of the block staged through LDS at a time
+constexpr int kMaxK3 = 384;    // K <= 128: A operands held in registers
+
 __global__ void __launch_bounds__(256) phase_gemm_mfma_kernel(
     const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin, int32_t K3,
     const uint16_t* bt_hi, const uint16_t* bt_lo, int32_t W, float* phase) {
-  __shared__ __attribute__((aligned(16))) uint16_t sAh[kTileS][kChunk + 8];
-  __shared__ __attribute__((aligned(16))) uint16_t sAl[kTileS][kChunk + 8];
+  __shared__ __attribute__((aligned(16))) uint16_t sAh[kTileS][kStageK3 + 8];
+  __shared__ __attribute__((aligned(16))) uint16_t sAl[kTileS][kStageK3 + 8];
+  // one k-chunk of the waveform block, shared by the four waves
   __shared__ __attribute__((aligned(16))) uint16_t sBh[kTileW][kChunk + 8];
   __shared__ __attribute__((aligned(16))) uint16_t sBl[kTileW][kChunk + 8];
   const int64_t s0 = (int64_t)blockIdx.x * kTileS;
-  const int w0 = blockIdx.y * kTileW;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  floatx4 acc[4];
+  const int nkc = K3 / kChunk;
+  const int r = wave * 16 + (lane & 15);
+  const int kl = 8 * (lane >> 4);
+  bf16x8 ah[kMaxK3 / kChunk], al[kMaxK3 / kChunk];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k0 = 0; k0 < K3; k0 += kChunk) {
-    // A: rec rows k0..k0+31, slots s0..s0+63 (coalesced), split on the way in
+  for (int st = 0; st < kMaxK3 / kStageK3; ++st) {  // stage up to 192 rows at a time
+    const int kh = st * kStageK3;  // (unrolled: register indices stay static)
+    if (kh < K3) {
+      const int rows = (K3 - kh < kStageK3) ? K3 - kh : kStageK3;
+      for (int e = tid; e < rows * kTileS; e += 256) {
+        const int kk = e / kTileS, ss = e % kTileS;
+        const int64_t s = s0 + ss;
+        const float v = (s < nslots) ? rec[(int64_t)(kh + kk) * ld + s] : 0.0f;
+        const uint16_t h = bf16_bits(v);
+        sAh[ss][kk] = h;
+        sAl[ss][kk] = bf16_bits(v - bf16_float(h));
+      }
+      __syncthreads();
 #pragma unroll
-    for (int q = 0; q < (kChunk * kTileS) / 256; ++q) {
-      const int e = q * 256 + tid;
-      const int kk = e / kTileS, ss = e % kTileS;
-      const int64_t s = s0 + ss;
-      const float v = (s < nslots) ? rec[(int64_t)(k0 + kk) * ld + s] : 0.0f;
-      const uint16_t h = bf16_bits(v);
-      sAh[ss][kk] = h;
-      sAl[ss][kk] = bf16_bits(v - bf16_float(h));
+      for (int j = 0; j < kStageK3 / kChunk; ++j) {
+        const int kc = st * (kStageK3 / kChunk) + j;
+        if (kc < nkc) {
+          __builtin_memcpy(&ah[kc], &sAh[r][j * kChunk + kl], 16);
+          __builtin_memcpy(&al[kc], &sAl[r][j * kChunk + kl], 16);
+        }
+      }
+      __syncthreads();
     }
-    // B: waveforms w0..w0+63, k0..k0+31 (contiguous per waveform)
-#pragma unroll
-    for (int q = 0; q < (kChunk * kTileW) / 256; ++q) {
-      const int e = q * 256 + tid;
-      const int ww = e / kChunk, kk = e % kChunk;
-      const int w = w0 + ww;
-      const int64_t src = (int64_t)w * K3 + k0 + kk;
-      sBh[ww][kk] = (w < W) ? bt_hi[src] : (uint16_t)0;
-      sBl[ww][kk] = (w < W) ? bt_lo[src] : (uint16_t)0;
-    }
-    __syncthreads();
-    const int r = wave * 16 + (lane & 15);
-    const int kb = 8 * (lane >> 4);
-    bf16x8 ah, al;
-    __builtin_memcpy(&ah, &sAh[r][kb], 16);
-    __builtin_memcpy(&al, &sAl[r][kb], 16);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int col = c * 16 + (lane & 15);
-      bf16x8 bh, bl;
-      __builtin_memcpy(&bh, &sBh[col][kb], 16);
-      __builtin_memcpy(&bl, &sBl[col][kb], 16);
-      acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[c], 0, 0, 0);
-    }
-    __syncthreads();
   }
-  // D: column lane % 16 of each tile, rows 4 (lane / 16) + i of the wave's 16
+  // the wave's 16 slots -> spins (row r of the tile: slot s0 + r)
+  int32_t spin_row = -1;
+  if (lane < 16) {
+    const int64_t s = s0 + wave * 16 + lane;
+    spin_row = (s < nslots) ? slot_spin[s] : -1;
+  }
+  float(*sC)[kTileW + 1] = reinterpret_cast<float(*)[kTileW + 1]>(&sAh[0][0]);
+  for (int w0 = 0; w0 < W; w0 += kTileW) {
+    floatx4 acc[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int64_t s = s0 + wave * 16 + 4 * (lane >> 4) + i;
-    if (s >= nslots) continue;
-    const int32_t spin = slot_spin[s];
-    if (spin < 0) continue;
+    for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int w = w0 + c * 16 + (lane & 15);
-      if (w < W) phase[(int64_t)spin * W + w] += acc[c][i];
+    for (int kc = 0; kc < kMaxK3 / kChunk; ++kc) {
+      if (kc < nkc) {
+      {  // 64 waveforms x 32 k, hi and lo: 16 bytes per thread each
+        const int ww = tid >> 2, q = (tid & 3) * 8;
+        const int w = w0 + ww;
+        bf16x8 vh = {}, vl = {};
+        if (w < W) {
+          const int64_t o = (int64_t)w * K3 + kc * kChunk + q;
+          __builtin_memcpy(&vh, &bt_hi[o], 16);
+          __builtin_memcpy(&vl, &bt_lo[o], 16);
+        }
+        __builtin_memcpy(&sBh[ww][q], &vh, 16);
+        __builtin_memcpy(&sBl[ww][q], &vl, 16);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int col = c * 16 + (lane & 15);
+        bf16x8 bh, bl;
+        __builtin_memcpy(&bh, &sBh[col][kl], 16);
+        __builtin_memcpy(&bl, &sBl[col][kl], 16);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bh, acc[c], 0, 0, 0);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bl, acc[c], 0, 0, 0);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kc], bh, acc[c], 0, 0, 0);
+      }
+      __syncthreads();
+      }
     }
+    // D (column lane % 16 of each 16x16 tile, rows 4 (lane / 16) + i) goes
+    // through LDS so that each phase row is updated by 64 lanes at once:
+    // 256 contiguous bytes per read-modify-write instead of 16 scattered
+    // 64-byte pieces.  Each wave only touches its own 16 rows.
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sC[wave * 16 + 4 * (lane >> 4) + i][c * 16 + (lane & 15)] = acc[c][i];
+    __syncthreads();
+    const int w = w0 + lane;
+    float old[16];  // all 16 row loads in flight before the first add
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+      const int32_t sp = __shfl(spin_row, rr, 64);
+      old[rr] = (sp >= 0 && w < W) ? phase[(int64_t)sp * W + w] : 0.0f;
+    }
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+      const int32_t sp = __shfl(spin_row, rr, 64);
+      if (sp >= 0 && w < W) phase[(int64_t)sp * W + w] = old[rr] + sC[wave * 16 + rr][lane];
+    }
+    __syncthreads();
   }
 }
 
@@ -244,8 +287,9 @@ int nz_phase_gemm(const float* rec, int64_t ld, int64_t nslots, const int32_t* s
   const int32_t K3 = 3 * K;
   if (mode == 0) {
     if (!bt_hi || !bt_lo) return fail(-1, "nz_phase_gemm: MFMA mode needs bt_hi and bt_lo");
-    if (K3 % kChunk) return fail(-1, "nz_phase_gemm: MFMA mode needs K a multiple of 32");
-    dim3 grid((unsigned)((nslots + kTileS - 1) / kTileS), (unsigned)((W + kTileW - 1) / kTileW));
+    if (K3 % kChunk || K3 > kMaxK3)
+      return fail(-1, "nz_phase_gemm: MFMA mode needs K a multiple of 32 and at most 128");
+    dim3 grid((unsigned)((nslots + kTileS - 1) / kTileS));
     hipLaunchKernelGGL(phase_gemm_mfma_kernel, grid, dim3(256), 0, (hipStream_t)stream, rec, ld,
                        nslots, slot_spin, K3, bt_hi, bt_lo, W, phase);
   } else {

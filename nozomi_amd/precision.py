@@ -65,8 +65,8 @@ class PrecisionRun:
         self.N = x0.shape[1]
         self.W, self.T = g.shape
         self.K = int(K)
-        if mode == "mfma" and self.K % 32:
-            raise ValueError("the MFMA contraction needs K a multiple of 32")
+        if mode == "mfma" and (self.K % 32 or self.K > 128):
+            raise ValueError("the MFMA contraction needs K a multiple of 32, at most 128")
         self.mode = 0 if mode == "mfma" else 1
         self.seed, self.dt = int(seed), float(dt)
         d = self.device
