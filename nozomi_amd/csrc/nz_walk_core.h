@@ -431,11 +431,11 @@ NZ_HD inline void lane_load_cells(const Grid& g, Lane& s) {
 
 NZ_HD inline void draw_jump(uint64_t seed, uint32_t spin, uint32_t step, uint32_t draw, double ds,
                             double j[3]) {
-  double n0, n1, n2;
-  normals3_d(seed, kStreamWalk, spin, step, draw, n0, n1, n2);
-  j[0] = ds * n0;
-  j[1] = ds * n1;
-  j[2] = ds * n2;
+  float n0, n1, n2;
+  normals3(seed, kStreamWalk, spin, step, draw, n0, n1, n2);
+  j[0] = ds * (double)n0;
+  j[1] = ds * (double)n1;
+  j[2] = ds * (double)n2;
 }
 
 // Progress of one spin through its current time step.

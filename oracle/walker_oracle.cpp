@@ -14,7 +14,9 @@
 // Pinning (what checks each function against something other than itself):
 //   orc_philox4x32_10  - Random123 known-answer vectors and rocRAND's
 //                        open-source host implementation (tests/test_oracle_rng.py)
-//   orc_normals        - moments of 1e6 draws, exact 23-bit uniform grid
+//   orc_normals        - moments of 1e6 draws, exact 23-bit uniform grid,
+//                        FP64 libm Box-Muller on the same uniforms, bit
+//                        equality with the product's host build
 //   orc_step_inside / orc_step_outside - closed-form ray/sphere hits on
 //                        constructed rays, tangents, tol-band branches, and
 //                        brute-force bisection (tests/test_oracle_primitives.py)
@@ -73,7 +75,53 @@ inline void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 }
 
 // 23-bit uniform on (0,1): ((x >> 9) + 1/2) * 2^-23, exact in binary32.
-inline double unif23(uint32_t x) { return ((double)(x >> 9) + 0.5) * (1.0 / 8388608.0); }
+inline float unif23(uint32_t x) { return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f); }
+
+// The walk's normals are a binary32 Box-Muller whose every operation is
+// correctly rounded (this file is built with -ffp-contract=off), so the
+// walker computes the same bits:
+//   ln u: u = m 2^e, m folded into [1/sqrt2, sqrt2), s = (m-1)/(m+1),
+//         ln u = e ln2 + 2 (s + s^3/3 + s^5/5 + s^7/7 + s^9/9)  (atanh series);
+//   sin/cos(2 pi v): v = q/4 + a, |a| <= 1/8, Taylor series of sin(2 pi a)
+//         to degree 9 and cos(2 pi a) to degree 10, then the quadrant q.
+// The coefficients are the binary32 roundings of 1/(2k+1) and of
+// (-1)^k (2 pi)^n / n!; ln 2 = 0.693145752 + 1.42860677e-6.
+inline float ln_u32(float u) {
+  int e = 0;
+  float m = std::frexp(u, &e) * 2.0f;  // u = m 2^(e-1), m in [1, 2)
+  e -= 1;
+  if (m > 1.41421354f) {
+    m *= 0.5f;
+    e += 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float z = s * s;
+  float p = std::fma(z, 0.111111112f, 0.142857149f);
+  p = std::fma(z, p, 0.200000003f);
+  p = std::fma(z, p, 0.333333343f);
+  p = std::fma(z, p, 1.0f);
+  const float fe = (float)e;
+  return std::fma(fe, 0.693145752f, std::fma(fe, 1.42860677e-06f, (2.0f * s) * p));
+}
+
+inline void sincos_2pi32(float v, float& sn, float& cs) {
+  const float q = std::floor(v * 4.0f + 0.5f);
+  const float a = v - q * 0.25f;
+  const float z = a * a;
+  float ps = std::fma(z, 42.0586929f, -76.7058563f);
+  ps = std::fma(z, ps, 81.6052475f);
+  ps = std::fma(z, ps, -41.3417015f);
+  ps = std::fma(z, ps, 6.28318548f);
+  const float sa = a * ps;
+  float pc = std::fma(z, -26.4262562f, 60.2446404f);
+  pc = std::fma(z, pc, -85.4568176f);
+  pc = std::fma(z, pc, 64.9393921f);
+  pc = std::fma(z, pc, -19.7392082f);
+  const float ca = std::fma(z, pc, 1.0f);
+  const int quad = ((int)q) & 3;
+  sn = quad == 0 ? sa : quad == 1 ? ca : quad == 2 ? -sa : -ca;
+  cs = quad == 0 ? ca : quad == 1 ? -sa : quad == 2 ? -ca : sa;
+}
 
 // Three standard normals for one (seed, stream, spin, step, draw) counter:
 // Box-Muller on (u0,u1) gives n0,n1; on (u2,u3) gives n2.
@@ -81,13 +129,14 @@ inline void normals3(uint64_t seed, uint32_t stream, uint32_t spin, uint32_t ste
                      uint32_t draw, double out[3]) {
   uint32_t c[4] = {draw, step, spin, stream};
   philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  const double u0 = unif23(c[0]), u1 = unif23(c[1]);
-  const double u2 = unif23(c[2]), u3 = unif23(c[3]);
-  const double r0 = std::sqrt(-2.0 * std::log(u0));
-  const double r1 = std::sqrt(-2.0 * std::log(u2));
-  out[0] = r0 * std::cos(2.0 * M_PI * u1);
-  out[1] = r0 * std::sin(2.0 * M_PI * u1);
-  out[2] = r1 * std::cos(2.0 * M_PI * u3);
+  const float r0 = std::sqrt(-2.0f * ln_u32(unif23(c[0])));
+  const float r1 = std::sqrt(-2.0f * ln_u32(unif23(c[2])));
+  float s0, c0, s1, c1;
+  sincos_2pi32(unif23(c[1]), s0, c0);
+  sincos_2pi32(unif23(c[3]), s1, c1);
+  out[0] = (double)(r0 * c0);
+  out[1] = (double)(r0 * s0);
+  out[2] = (double)(r1 * c1);
 }
 
 // ------------------------------------------------------ collision primitives

@@ -38,6 +38,8 @@ def lib():
                                  _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
         L.nzcpu_state_f64.argtypes = [_vp, _vp, _vp]
         L.nzcpu_info.argtypes = [_vp, _vp]
+        L.nzcpu_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.c_uint32, ctypes.c_int, _vp]
         _lib = L
     return _lib
 

@@ -43,6 +43,14 @@ void* nzcpu_create(const double* spheres, int64_t S, int32_t nstruct, const doub
 
 void nzcpu_destroy(void* h) { delete (CpuWalker*)h; }
 
+// the walk's normals (host build of nz::normals3), n draws from draw0
+void nzcpu_normals(uint64_t seed, uint32_t stream, uint32_t spin, uint32_t step, uint32_t draw0,
+                   int n, float* out) {
+  for (int k = 0; k < n; ++k)
+    nz::normals3(seed, stream, spin, step, draw0 + (uint32_t)k, out[3 * k], out[3 * k + 1],
+                 out[3 * k + 2]);
+}
+
 void nzcpu_info(void* h, double* out /* ncell[3], cell[3], margin, ncand_mean_box */) {
   const nz_substrate_info_t& inf = ((CpuWalker*)h)->hg.info;
   for (int a = 0; a < 3; ++a) { out[a] = inf.ncell[a]; out[3 + a] = inf.cell[a]; }

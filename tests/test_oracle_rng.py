@@ -74,3 +74,37 @@ def test_uniform_grid_is_exact(oracle_lib):
     u = ((x >> 9).astype(np.float64) + 0.5) * 2.0 ** -23
     assert np.array_equal(u.astype(np.float32).astype(np.float64), u)
     assert u.min() > 0 and u.max() < 1
+
+
+def test_walk_normals_bit_identical_to_product(oracle_lib):
+    """The walker's normals (host build of the product's nz_rng.h) equal the
+    oracle's restatement bit for bit (binary32, correctly rounded operations
+    only), across seeds, spins, steps and draw indices."""
+    from tests.cpu_build import lib as cpu_lib
+    L = cpu_lib()
+    rng = np.random.default_rng(3)
+    for _ in range(40):
+        seed = int(rng.integers(0, 2**63))
+        spin, step, draw0 = (int(v) for v in rng.integers(0, 2**31, 3))
+        n = 256
+        got = np.zeros((n, 3), dtype=np.float32)
+        L.nzcpu_normals(seed, 0, spin, step, draw0, n, got.ctypes.data)
+        ref = oracle_lib.normals(seed, 0, spin, step, draw0, n)
+        assert np.array_equal(got.astype(np.float64), ref)
+
+
+def test_walk_normals_accuracy(oracle_lib):
+    """The binary32 polynomial log and sin/cos against FP64 libm on the same
+    uniforms: the normals are within a few float ulps of exact Box-Muller."""
+    n = 20000
+    out = oracle_lib.normals(12345, 0, 7, 9, 0, n)
+    # regenerate the uniforms: Philox words -> 23-bit grid
+    words = np.array([oracle_lib.philox4x32_10([k, 9, 7, 0], [12345 & 0xFFFFFFFF, 12345 >> 32])
+                      for k in range(n)], dtype=np.uint64)
+    u = ((words >> 9).astype(np.float64) + 0.5) / 2**23
+    r0 = np.sqrt(-2 * np.log(u[:, 0]))
+    r1 = np.sqrt(-2 * np.log(u[:, 2]))
+    exact = np.stack([r0 * np.cos(2 * np.pi * u[:, 1]), r0 * np.sin(2 * np.pi * u[:, 1]),
+                      r1 * np.cos(2 * np.pi * u[:, 3])], 1)
+    err = np.abs(out - exact) / np.maximum(1.0, np.abs(exact))
+    assert err.max() < 2e-6, err.max()
