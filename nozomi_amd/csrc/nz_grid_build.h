@@ -18,11 +18,18 @@
 
 #include "nz_internal.h"
 
+// Edge (in cells) of the regions within which cells are ordered by list
+// length for the spatial sort.
+#ifndef NZ_RANK_REGION
+#define NZ_RANK_REGION 4
+#endif
+
 namespace nz {
 
 struct HostGrid {
   Grid g;  // scalar fields set; pointers bound by bind_host() or by the upload
   std::vector<int32_t> start;
+  std::vector<int32_t> rank;
   std::vector<float4> cand;  // a, b records interleaved
   std::vector<double4> sph;
   std::vector<int32_t> sid;
@@ -31,6 +38,7 @@ struct HostGrid {
 
   void bind_host() {
     g.cell_start = start.data();
+    g.cell_rank = rank.data();
     g.cand = cand.data();
     g.sph = sph.data();
     g.sph_sid = sid.data();
@@ -148,6 +156,30 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
   }
   out.start.resize(ncells + 1);
   for (int64_t c = 0; c <= ncells; ++c) out.start[c] = (int32_t)cnt[c];
+  {  // sort ranks of the in-box cells: R^3-cell regions, then list length
+    const int R = NZ_RANK_REGION;
+    const int nx = g.ncell[0], ny = g.ncell[1], nz = g.ncell[2];
+    const int rx = (nx + R - 1) / R, ry = (ny + R - 1) / R;
+    const int64_t nbox = (int64_t)nx * ny * nz;
+    std::vector<int64_t> order(nbox);
+    std::vector<int64_t> region(nbox), len(nbox);
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const int64_t c = ((int64_t)z * ny + y) * nx + x;
+          const size_t lc = ((size_t)(z + 1) * g.nloc[1] + (y + 1)) * g.nloc[0] + (x + 1);
+          region[c] = ((int64_t)(z / R) * ry + y / R) * rx + x / R;
+          len[c] = cnt[lc + 1] - cnt[lc];
+          order[c] = c;
+        }
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+      if (region[a] != region[b]) return region[a] < region[b];
+      if (len[a] != len[b]) return len[a] < len[b];
+      return a < b;
+    });
+    out.rank.assign(nbox, 0);
+    for (int64_t k = 0; k < nbox; ++k) out.rank[order[k]] = (int32_t)k;
+  }
   out.sph.resize(S);
   out.sid.resize(S);
   for (int64_t i = 0; i < S; ++i) {

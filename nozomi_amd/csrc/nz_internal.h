@@ -41,6 +41,10 @@ struct Grid {
   //   b = (r^2, (r-tol)^2, (r+tol)^2, sphere index as int bits)
   // A list holds at most 255 entries.
   const int32_t* cell_start;  // [ncells + 1]
+  // sort position of each in-box cell (index (z ncell_y + y) ncell_x + x):
+  // cells grouped in 4x4x4 regions, ordered by list length within a region,
+  // so a wavefront's spins tend to scan lists of similar length
+  const int32_t* cell_rank;
   const float4* cand;         // [2 * entries + 2]: a padding record at the end
   // every sphere, absolute, FP64: the exact re-solves and the rare sub-step
   // that leaves the grid's lists

@@ -80,7 +80,8 @@ int nz_substrate_create(const double* spheres, int64_t S, int32_t nstruct, const
   s->info.device_bytes = 0;
   Grid& g = s->g;
   int rc = 0;
-  if ((rc = upload(s, hg.start, &g.cell_start)) || (rc = upload(s, hg.cand, &g.cand)) ||
+  if ((rc = upload(s, hg.start, &g.cell_start)) || (rc = upload(s, hg.rank, &g.cell_rank)) ||
+      (rc = upload(s, hg.cand, &g.cand)) ||
       (rc = upload(s, hg.sph, &g.sph)) ||
       (rc = upload(s, hg.sid, &g.sph_sid)) || (rc = upload(s, hg.D, &g.D)) ||
       (rc = upload(s, hg.T2, &g.T2))) {

@@ -309,10 +309,11 @@ __global__ void __launch_bounds__(256) sort_keys_kernel(const Grid g, const Slot
                                                         int32_t* keys, int32_t* counts) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
-  // intra-structure spins first, then extra-structure spins; cell order
-  // within each class (spins with similar collision rates share wavefronts)
+  // intra-structure spins first, then extra-structure spins; within a class
+  // the cells' sort ranks (regions, then list length): spins with similar
+  // collision rates and candidate counts share wavefronts
   const int64_t ncells = (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
-  const int32_t k = valid ? wrapped_cell(g, st.ucell, n, i) +
+  const int32_t k = valid ? g.cell_rank[wrapped_cell(g, st.ucell, n, i)] +
                                 (st.sid[i] >= g.nstruct ? (int32_t)ncells : 0)
                           : -1;
   if (valid) keys[i] = k;
@@ -711,7 +712,7 @@ __global__ void __launch_bounds__(256) batch_keys_kernel(const SimDesc* descs,
         c[a] = w < 0 ? w + g.ncell[a] : w;
       }
       const int64_t cell = ((int64_t)c[2] * g.ncell[1] + c[1]) * g.ncell[0] + c[0];
-      key = d.key0 + cell + (st.sid[i] >= g.nstruct ? d.ncells : 0);
+      key = d.key0 + g.cell_rank[cell] + (st.sid[i] >= g.nstruct ? d.ncells : 0);
     }
   }
   keys[i] = (int32_t)key;
