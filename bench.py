@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--box", type=float, default=60.0)
     ap.add_argument("--cell", type=float, default=0.0)
     ap.add_argument("--margin", type=float, default=0.0)
-    ap.add_argument("--sort", type=int, default=64, help="steps between spatial re-sorts")
+    ap.add_argument("--sort", type=int, default=96, help="steps between spatial re-sorts")
     ap.add_argument("--cpu-sample", type=int, default=200000,
                     help="spins of the CPU-baseline subsample (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=5000)
@@ -273,7 +273,7 @@ def forms(args):
     cache.prefetch(reqs)
     cap = sum(-(-r.n // 256) * 256 for r in reqs)
     run_serial(reqs[:4], cache=cache)  # warms the kernels
-    got, t_b = run_batched(reqs, capacity=cap, k=64, cache=cache)
+    got, t_b = run_batched(reqs, capacity=cap, k=96, cache=cache)
     alone, t_s = run_serial(reqs, cache=cache)
     diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
     ss = total_spin_steps(reqs)

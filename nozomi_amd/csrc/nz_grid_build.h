@@ -65,7 +65,7 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
       return -1;
     }
   }
-  if (cell_size <= 0) cell_size = 1.5;
+  if (cell_size <= 0) cell_size = 1.25;
   if (margin <= 0) margin = 0.8;
   Grid& g = out.g;
   std::memset(&g, 0, sizeof(g));

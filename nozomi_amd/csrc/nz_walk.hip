@@ -40,7 +40,7 @@ struct Walker {
   int32_t* partial = nullptr; // [scan tiles] tile sums of the key-count scan
   float* pos_tmp = nullptr;   // [3,n] seeding output
   int64_t nkeys = 0;          // sort key range: 2 x cells (intra / extra)
-  int32_t sort_interval = 64;
+  int32_t sort_interval = 96;
   int32_t xcd_chunk = 64;     // NZ_WALK_XCD_CHUNK in the environment overrides it
   int32_t since_sort = 1 << 30;
   int64_t step = 0;

@@ -245,7 +245,7 @@ def serve_trace(server, requests, arrivals, poll=0.0):
     return server.results, time.perf_counter() - t0
 
 
-def run_batched(requests, capacity, k=64, launches=2, device=None, graph=True, max_sims=256,
+def run_batched(requests, capacity, k=96, launches=2, device=None, graph=True, max_sims=256,
                 cache=None):
     """Every request submitted at once, run to completion by one Server.
     Returns (results, seconds)."""
