@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--precision-steps", type=int, default=2000,
                     help="steps of the precision-form measurement (1e4 = the full workload)")
     ap.add_argument("--precision-waveforms", type=int, default=1024)
+    ap.add_argument("--only", choices=("precision", "batching", "serving"), default=None,
+                    help="measure only this form and print its JSON object")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy a buffer of this many MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -83,6 +85,18 @@ def main():
         copy_probe(args.copy_probe)
         return
     import torch
+    if args.only:
+        if args.only == "precision":
+            from nozomi_amd.substrate import generate_beaded_axons, load_geometry
+            M, L = generate_beaded_axons(args.axons, args.box, seed=args.seed)
+            geom = load_geometry(M, L, D=2.0, T2=100.0)
+            sub = geom.device_substrate(cell_size=args.cell, margin=args.margin,
+                                        device=torch.device("cuda:0"))
+            print(json.dumps(precision_form(args, geom, sub)), flush=True)
+        else:
+            print(json.dumps((batching_form if args.only == "batching" else serving_form)(args)),
+                  flush=True)
+        return
 
     from nozomi_amd import runtime as rt
     from nozomi_amd.protocol import PGSEShells, fibonacci_hemisphere
@@ -258,12 +272,13 @@ def precision_form(args, geom, sub):
 def forms(args):
     """Batched and served forms against back-to-back plain walker launches of
     the same simulations (each result checked against its run alone)."""
+    return {"batching": batching_form(args), "serving": serving_form(args)}
+
+
+def batching_form(args):
     import torch
-    from nozomi_amd.serve import (Server, fifo_latencies, latency_stats, run_batched, run_serial,
-                                  serve_trace)
-    from nozomi_amd.serve import SubstrateCache
-    from nozomi_amd.workloads import batching_mix, pregenerate, serving_mix, total_spin_steps
-    res = {}
+    from nozomi_amd.serve import SubstrateCache, run_batched, run_serial
+    from nozomi_amd.workloads import batching_mix, pregenerate, total_spin_steps
     dev = torch.device("cuda:0")
     # batching: 256 independent simulations of uneven size in one pool; their
     # substrates are generated and uploaded before timing (resident in HBM)
@@ -277,13 +292,17 @@ def forms(args):
     alone, t_s = run_serial(reqs, cache=cache)
     diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
     ss = total_spin_steps(reqs)
-    res["batching"] = {
+    return {
         "sims": len(reqs), "spin_steps": ss, "step_scale": args.forms_steps,
         "batched_sims_per_s": len(reqs) / t_b, "serial_sims_per_s": len(reqs) / t_s,
         "speedup_vs_serial": t_s / t_b, "batched_spin_steps_per_s": ss / t_b,
         "serial_spin_steps_per_s": ss / t_s, "max_signal_diff_vs_alone": diff,
     }
-    del cache
+
+
+def serving_form(args):
+    from nozomi_amd.serve import (Server, fifo_latencies, latency_stats, run_serial, serve_trace)
+    from nozomi_amd.workloads import pregenerate, serving_mix, total_spin_steps
     # serving: Poisson arrivals at 50 req/s, continuous batching, HBM admission;
     # device substrates are built when a request first needs one
     reqs, arr = serving_mix(rate=50.0, duration=args.serve_seconds, seed=args.seed + 1,
@@ -298,7 +317,7 @@ def forms(args):
     diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
     ss = total_spin_steps(reqs)
     lat_serial = fifo_latencies(arr, [alone[r.rid].latency for r in reqs])
-    res["serving"] = {
+    return {
         "requests": len(reqs), "rate_per_s": 50.0, "arrival_window_s": args.serve_seconds,
         "step_scale": args.forms_steps, "offered_spin_steps_per_s": ss / args.serve_seconds,
         "makespan_s": makespan, "served_sims_per_s": len(reqs) / makespan,
@@ -309,7 +328,6 @@ def forms(args):
         "substrates_built": built, "substrates_reused": reused,
         "max_signal_diff_vs_alone": diff,
     }
-    return res
 
 
 def measured_traffic(n, steps_per_launch, info):
