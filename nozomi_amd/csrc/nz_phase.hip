@@ -62,102 +62,99 @@ __global__ void __launch_bounds__(256) waveform_block_kernel(const float* g, int
 constexpr int kTileS = 64;   // slots per workgroup tile
 constexpr int kTileW = 64;   // waveforms per workgroup tile
 constexpr int kChunk = 32;   // k per MFMA (16x16x32 BF16)
+constexpr int kMaxK3 = 384;  // K <= 128: A operands held in registers
 
-// phase[spin(s), w] += sum_k rec[k][s] * Bt[w][k] over one block of K3 rows,
-// 16x16x32 BF16 MFMAs, split hi/lo (three products).  Workgroup: 4 waves and
-// 64 slots; wave q owns slots [16q, 16q + 16) and walks every waveform in
-// tiles of 64.  The block's displacements are read from HBM once: staged
-// through LDS (coalesced rows -> per-lane k-contiguous vectors), split, and
-// kept in registers as the A operands of all waveform tiles; the B operands
-// (the waveform block, a few hundred KB shared by all workgroups) come from L2.
+// phase[spin(s), w] += sum_k rec[k][s] * Bt[w][k] over one block of K3 =
+// 32 NKC rows, 16x16x32 BF16 MFMAs, split hi/lo (three products).
+// Workgroup: 4 waves and 64 slots; wave q owns slots [16q, 16q + 16) and
+// walks every waveform in tiles of 64.
+// - A (the block's displacements) is read from HBM once, straight into the
+//   registers that hold it as MFMA A operands for all waveform tiles (lane l:
+//   slot l % 16 of the wave, k = 8 (l / 16) .. + 8 of each 32-k chunk; 16
+//   lanes read 64 contiguous bytes of a record row).
+// - B (the waveform block, a few hundred KB shared by all workgroups, served
+//   from L2) moves through two LDS buffers, one 32-k chunk of one waveform
+//   tile at a time: the next chunk's global load is in flight while the
+//   current chunk's MFMAs run, and one barrier per chunk suffices.
+// - D goes through LDS so each phase row is read-modified-written by 64 lanes
+//   at once (256 contiguous bytes); each wave touches only its own 16 rows.
+// 37 KB of LDS and 168 VGPRs: three workgroups per CU.
 // MFMA operand layout (lane l of 64): A row l % 16, k 8 (l / 16) .. + 8;
 // B column l % 16, the same k; D column l % 16, rows 4 (l / 16) .. + 4.
-constexpr int kStageK3 = 192;  // rows of the block staged through LDS at a time
-constexpr int kMaxK3 = 384;    // K <= 128: A operands held in registers
-
-__global__ void __launch_bounds__(256) phase_gemm_mfma_kernel(
-    const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin, int32_t K3,
-    const uint16_t* bt_hi, const uint16_t* bt_lo, int32_t W, float* phase) {
-  __shared__ __attribute__((aligned(16))) uint16_t sAh[kTileS][kStageK3 + 8];
-  __shared__ __attribute__((aligned(16))) uint16_t sAl[kTileS][kStageK3 + 8];
-  // one k-chunk of the waveform block, shared by the four waves
-  __shared__ __attribute__((aligned(16))) uint16_t sBh[kTileW][kChunk + 8];
-  __shared__ __attribute__((aligned(16))) uint16_t sBl[kTileW][kChunk + 8];
+template <int NKC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin,
+                       const uint16_t* bt_hi, const uint16_t* bt_lo, int32_t W, float* phase) {
+  constexpr int K3 = NKC * kChunk;
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][kTileW][kChunk + 8];  // [buf][hi, lo]
+  __shared__ float sC[kTileS][kTileW + 1];
   const int64_t s0 = (int64_t)blockIdx.x * kTileS;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nkc = K3 / kChunk;
-  const int r = wave * 16 + (lane & 15);
   const int kl = 8 * (lane >> 4);
-  bf16x8 ah[kMaxK3 / kChunk], al[kMaxK3 / kChunk];
+  const int64_t srow = s0 + wave * 16 + (lane & 15);
+  const bool row_ok = srow < nslots;
+  bf16x8 ah[NKC], al[NKC];
 #pragma unroll
-  for (int st = 0; st < kMaxK3 / kStageK3; ++st) {  // stage up to 192 rows at a time
-    const int kh = st * kStageK3;  // (unrolled: register indices stay static)
-    if (kh < K3) {
-      const int rows = (K3 - kh < kStageK3) ? K3 - kh : kStageK3;
-      for (int e = tid; e < rows * kTileS; e += 256) {
-        const int kk = e / kTileS, ss = e % kTileS;
-        const int64_t s = s0 + ss;
-        const float v = (s < nslots) ? rec[(int64_t)(kh + kk) * ld + s] : 0.0f;
-        const uint16_t h = bf16_bits(v);
-        sAh[ss][kk] = h;
-        sAl[ss][kk] = bf16_bits(v - bf16_float(h));
-      }
-      __syncthreads();
+  for (int kc = 0; kc < NKC; ++kc) {
+    uint16_t hb[8], lb[8];
 #pragma unroll
-      for (int j = 0; j < kStageK3 / kChunk; ++j) {
-        const int kc = st * (kStageK3 / kChunk) + j;
-        if (kc < nkc) {
-          __builtin_memcpy(&ah[kc], &sAh[r][j * kChunk + kl], 16);
-          __builtin_memcpy(&al[kc], &sAl[r][j * kChunk + kl], 16);
-        }
-      }
-      __syncthreads();
+    for (int j = 0; j < 8; ++j) {
+      const float v = row_ok ? rec[(int64_t)(kc * kChunk + kl + j) * ld + srow] : 0.0f;
+      hb[j] = bf16_bits(v);
+      lb[j] = bf16_bits(v - bf16_float(hb[j]));
     }
+    __builtin_memcpy(&ah[kc], hb, 16);
+    __builtin_memcpy(&al[kc], lb, 16);
   }
-  // the wave's 16 slots -> spins (row r of the tile: slot s0 + r)
+  // the wave's 16 slots -> spins
   int32_t spin_row = -1;
   if (lane < 16) {
     const int64_t s = s0 + wave * 16 + lane;
     spin_row = (s < nslots) ? slot_spin[s] : -1;
   }
-  float(*sC)[kTileW + 1] = reinterpret_cast<float(*)[kTileW + 1]>(&sAh[0][0]);
+  // this thread's 16-byte piece of every B chunk: waveform ww, k q .. q + 8
+  const int ww = tid >> 2, q = (tid & 3) * 8;
+  bf16x8 nh = {}, nl = {};  // the next chunk, in flight
+  if (ww < W) {
+    __builtin_memcpy(&nh, &bt_hi[(int64_t)ww * K3 + q], 16);
+    __builtin_memcpy(&nl, &bt_lo[(int64_t)ww * K3 + q], 16);
+  }
+  int buf = 0;
   for (int w0 = 0; w0 < W; w0 += kTileW) {
     floatx4 acc[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int kc = 0; kc < kMaxK3 / kChunk; ++kc) {
-      if (kc < nkc) {
-      {  // 64 waveforms x 32 k, hi and lo: 16 bytes per thread each
-        const int ww = tid >> 2, q = (tid & 3) * 8;
-        const int w = w0 + ww;
-        bf16x8 vh = {}, vl = {};
-        if (w < W) {
-          const int64_t o = (int64_t)w * K3 + kc * kChunk + q;
-          __builtin_memcpy(&vh, &bt_hi[o], 16);
-          __builtin_memcpy(&vl, &bt_lo[o], 16);
+    for (int kc = 0; kc < NKC; ++kc) {
+      // buffer `buf` was last read two chunks ago, before the previous barrier
+      __builtin_memcpy(&sB[buf][0][ww][q], &nh, 16);
+      __builtin_memcpy(&sB[buf][1][ww][q], &nl, 16);
+      {
+        const int nw0 = (kc + 1 < NKC) ? w0 : w0 + kTileW;
+        const int nkc = (kc + 1 < NKC) ? kc + 1 : 0;
+        if (nw0 < W) {
+          nh = bf16x8{};
+          nl = bf16x8{};
+          if (nw0 + ww < W) {
+            const int64_t o = (int64_t)(nw0 + ww) * K3 + nkc * kChunk + q;
+            __builtin_memcpy(&nh, &bt_hi[o], 16);
+            __builtin_memcpy(&nl, &bt_lo[o], 16);
+          }
         }
-        __builtin_memcpy(&sBh[ww][q], &vh, 16);
-        __builtin_memcpy(&sBl[ww][q], &vl, 16);
       }
       __syncthreads();
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int col = c * 16 + (lane & 15);
         bf16x8 bh, bl;
-        __builtin_memcpy(&bh, &sBh[col][kl], 16);
-        __builtin_memcpy(&bl, &sBl[col][kl], 16);
+        __builtin_memcpy(&bh, &sB[buf][0][col][kl], 16);
+        __builtin_memcpy(&bl, &sB[buf][1][col][kl], 16);
         acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bh, acc[c], 0, 0, 0);
         acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bl, acc[c], 0, 0, 0);
         acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kc], bh, acc[c], 0, 0, 0);
       }
-      __syncthreads();
-      }
+      buf ^= 1;
     }
-    // D (column lane % 16 of each 16x16 tile, rows 4 (lane / 16) + i) goes
-    // through LDS so that each phase row is updated by 64 lanes at once:
-    // 256 contiguous bytes per read-modify-write instead of 16 scattered
-    // 64-byte pieces.  Each wave only touches its own 16 rows.
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -175,7 +172,6 @@ __global__ void __launch_bounds__(256) phase_gemm_mfma_kernel(
       const int32_t sp = __shfl(spin_row, rr, 64);
       if (sp >= 0 && w < W) phase[(int64_t)sp * W + w] = old[rr] + sC[wave * 16 + rr][lane];
     }
-    __syncthreads();
   }
 }
 
@@ -290,8 +286,12 @@ int nz_phase_gemm(const float* rec, int64_t ld, int64_t nslots, const int32_t* s
     if (K3 % kChunk || K3 > kMaxK3)
       return fail(-1, "nz_phase_gemm: MFMA mode needs K a multiple of 32 and at most 128");
     dim3 grid((unsigned)((nslots + kTileS - 1) / kTileS));
-    hipLaunchKernelGGL(phase_gemm_mfma_kernel, grid, dim3(256), 0, (hipStream_t)stream, rec, ld,
-                       nslots, slot_spin, K3, bt_hi, bt_lo, W, phase);
+    auto* kern = (K3 == 96)    ? phase_gemm_mfma_kernel<3>
+                 : (K3 == 192) ? phase_gemm_mfma_kernel<6>
+                 : (K3 == 288) ? phase_gemm_mfma_kernel<9>
+                               : phase_gemm_mfma_kernel<12>;
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, (hipStream_t)stream, rec, ld, nslots, slot_spin,
+                       bt_hi, bt_lo, W, phase);
   } else {
     if (!bt_f32) return fail(-1, "nz_phase_gemm: FP32 mode needs bt_f32");
     dim3 grid((unsigned)((nslots + 255) / 256), (unsigned)((W + 15) / 16));

@@ -15,9 +15,10 @@ pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
 
 
-def test_gemm_modes_against_fp64():
-    rng = np.random.default_rng(0)
-    K, N, W = 64, 1000, 200
+@pytest.mark.parametrize("K", [32, 64, 96, 128])
+def test_gemm_modes_against_fp64(K):
+    rng = np.random.default_rng(K)
+    N, W = 1000, 200
     rec = rng.normal(scale=5.0, size=(3 * K, N)).astype(np.float32)
     bt = rng.normal(scale=1e-3, size=(W, 3 * K)).astype(np.float32)
     slot_spin = rng.permutation(N).astype(np.int32)
