@@ -34,6 +34,7 @@ struct HostGrid {
   std::vector<double4> sph;
   std::vector<int32_t> sid;
   std::vector<double> D, T2;
+  std::vector<float> inv_T2;
   nz_substrate_info_t info;
 
   void bind_host() {
@@ -44,6 +45,7 @@ struct HostGrid {
     g.sph_sid = sid.data();
     g.D = D.data();
     g.T2 = T2.data();
+    g.inv_T2 = inv_T2.data();
   }
 };
 
@@ -188,6 +190,8 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
   }
   out.D.assign(D, D + nstruct + 1);
   out.T2.assign(T2, T2 + nstruct + 1);
+  out.inv_T2.resize(nstruct + 1);
+  for (int32_t k = 0; k <= nstruct; ++k) out.inv_T2[k] = (float)(1.0 / T2[k]);
 
   nz_substrate_info_t& inf = out.info;
   inf.nspheres = S;

@@ -53,6 +53,7 @@ struct Grid {
   // per-structure parameters, box last
   const double* D;
   const double* T2;
+  const float* inv_T2;  // (float)(1 / T2), the factor of the log T2 weight
 };
 
 struct Substrate {

@@ -84,7 +84,7 @@ int nz_substrate_create(const double* spheres, int64_t S, int32_t nstruct, const
       (rc = upload(s, hg.cand, &g.cand)) ||
       (rc = upload(s, hg.sph, &g.sph)) ||
       (rc = upload(s, hg.sid, &g.sph_sid)) || (rc = upload(s, hg.D, &g.D)) ||
-      (rc = upload(s, hg.T2, &g.T2))) {
+      (rc = upload(s, hg.T2, &g.T2)) || (rc = upload(s, hg.inv_T2, &g.inv_T2))) {
     release(s);
     delete s;
     return rc;

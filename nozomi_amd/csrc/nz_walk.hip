@@ -170,6 +170,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   bool started = false;
   StepState q;
   q.pending = -1;
+  q.ds_sid = -1.0f;
   // A spin only stops at a step boundary: the sub-step state of a step in
   // progress is not kept across launches.
   while (__any(t < P.target || started)) {

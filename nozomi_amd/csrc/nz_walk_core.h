@@ -442,6 +442,7 @@ NZ_HD inline void draw_jump(uint64_t seed, uint32_t spin, uint32_t step, uint32_
 struct StepState {
   double jr;    // remaining jump fraction (sim3d_kernel.cu:364)
   double ds;    // jump scale sqrt(2 D dt) of the step
+  float ds_sid; // structure id that ds belongs to (-1: none yet; dt is fixed)
   double j[3];  // current jump vector
   int niter;    // sub-steps done in this step
   int cs, ce;   // candidate list of the lane's cell
@@ -455,7 +456,10 @@ NZ_HD inline void step_begin(const Grid& g, const Lane& s, StepState& q, double 
   q.jr = 1.0;
   q.niter = 0;
   q.pending = 0;
-  q.ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
+  if (s.sid != q.ds_sid) {  // the structure changed since ds was computed
+    q.ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
+    q.ds_sid = s.sid;
+  }
   const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
   q.cs = g.cell_start[cid];
   q.ce = g.cell_start[cid + 1];
@@ -522,6 +526,7 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
     s.sid = acc.sid_new;  // structure changed: redo with the new id
     if (q.niter == 1) {
       q.ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
+      q.ds_sid = s.sid;
       draw_jump(seed, spin, step, 0u, q.ds, q.j);
     }
   }
@@ -537,15 +542,19 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
     const bool need_in = acc.kin >= 0 && acc.any_in && fi <= lim_in;
     const bool need_out = acc.kout >= 0 && acc.fout <= lim_out;
     if (acc.any_in && acc.kin < 0) fin = 0.0;
+    // A lane rarely needs both roots (a near-tie): the first pass solves the
+    // exit root or else the entry root, the second only the entry root of
+    // lanes that need both, so a wavefront usually runs the FP64 solve once.
 #pragma unroll 1
     for (int r = 0; r < 2; ++r) {
-      const bool want = (r == 0) ? need_in : need_out;
+      const bool in_root = (r == 0) && need_in;
+      const bool want = (r == 0) ? (need_in || need_out) : (need_in && need_out);
       if (want) {
-        const double4 c = g.sph[(r == 0) ? acc.kin : acc.kout];
-        const double v = root_d(r == 0, s.p[0] - (c.x - cell_lo(g, 0, s.l[0])),
+        const double4 c = g.sph[in_root ? acc.kin : acc.kout];
+        const double v = root_d(in_root, s.p[0] - (c.x - cell_lo(g, 0, s.l[0])),
                                 s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
                                 s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, q.j[0], q.j[1], q.j[2]);
-        if (r == 0) fin = v;
+        if (in_root) fin = v;
         else fout = v;
       }
     }
@@ -555,7 +564,7 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
   s.p[0] += jc * q.j[0];                             // :460-462
   s.p[1] += jc * q.j[1];
   s.p[2] += jc * q.j[2];
-  const float inv_t2 = (float)(1.0 / g.T2[(int)s.sid]);
+  const float inv_t2 = g.inv_T2[(int)s.sid];
   if (jc < q.jr) {  // scatter (:464-479)
     q.jr = sqrt(MAXd(0.0, q.jr * q.jr - jc * jc));
     if (do_t2) s.logsig = fmaf((float)(-jc * jc * dt), inv_t2, s.logsig);
@@ -577,19 +586,34 @@ NZ_HD inline void step_end(const Grid& g, Lane& s, float G, float Ga) {
   for (int a = 0; a < 3; ++a) {
     const double x = cell_lo(g, a, s.l[a]) + s.p[a];
     s.m[a] = fmaf(G, (float)x, s.m[a]);
-    int k = (int)floor(s.p[a] / g.hd[a]);
-    double r = s.p[a] - (double)k * g.hd[a];
-    if (r < 0.0) { r += g.hd[a]; --k; }
-    if (r >= g.hd[a]) { r -= g.hd[a]; ++k; }
+    // the cell holding the end point: k cells from l (r = p - k h in [0, h));
+    // a step rarely leaves the neighbouring cell, so the division is a
+    // fallback and the usual case is two comparisons
+    double r = s.p[a];
+    int k = 0;
+    if (r < 0.0) {
+      r += g.hd[a];
+      k = -1;
+    } else if (r >= g.hd[a]) {
+      r -= g.hd[a];
+      k = 1;
+    }
+    if (r < 0.0 || r >= g.hd[a]) {
+      k = (int)floor(s.p[a] / g.hd[a]);
+      r = s.p[a] - (double)k * g.hd[a];
+      if (r < 0.0) { r += g.hd[a]; --k; }
+      if (r >= g.hd[a]) { r -= g.hd[a]; ++k; }
+    }
     int nl = s.l[a] + k;
     if (nl == g.ncell[a] + 1 && r == 0.0) { --nl; --k; r = g.hd[a]; }  // x == L/2 stays
     s.p[a] = r;
     s.u[a] += k;
     // wrap: x > L/2  <=>  nl > ncell;  x < -L/2  <=>  nl < 1.  w = net number
-    // of +L wraps; each one shifts the phase by -L*G_area
+    // of +L wraps (one unless the step spans more than the box); each one
+    // shifts the phase by -L*G_area
     int w = 0;
-    if (nl > g.ncell[a]) w = (nl - 1) / g.ncell[a];
-    else if (nl < 1) w = -((g.ncell[a] - nl) / g.ncell[a]);
+    if (nl > g.ncell[a]) w = (nl <= 2 * g.ncell[a]) ? 1 : (nl - 1) / g.ncell[a];
+    else if (nl < 1) w = (nl > -g.ncell[a]) ? -1 : -((g.ncell[a] - nl) / g.ncell[a]);
     if (w != 0) {
       nl -= w * g.ncell[a];
       s.m[a] = fmaf(-(float)w * g.L[a], Ga, s.m[a]);
@@ -602,6 +626,7 @@ NZ_HD inline void step_end(const Grid& g, Lane& s, float G, float Ga) {
 NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin, uint32_t step,
                             double dt, bool do_t2, float G, float Ga) {
   StepState q;
+  q.ds_sid = -1.0f;
   step_begin(g, s, q, dt);
   bool done = false;
   while (!done) {
