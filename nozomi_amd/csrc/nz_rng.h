@@ -10,13 +10,11 @@
 // tests) take portable equivalents of the device intrinsics below.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define NZ_HD __host__ __device__
-__device__ __forceinline__ uint32_t nz_umulhi(uint32_t a, uint32_t b) { return __umulhi(a, b); }
 __device__ __forceinline__ float nz_rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float nz_sqrtf(float x) { return __builtin_amdgcn_sqrtf(x); }
 #else
 #include <cmath>
 #define NZ_HD __host__ __device__
-inline uint32_t nz_umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 inline float nz_rcpf(float x) { return 1.0f / x; }
 inline float nz_sqrtf(float x) { return std::sqrt(x); }
 #endif
@@ -29,13 +27,20 @@ struct u32x4 {
   uint32_t x, y, z, w;
 };
 
+// Both halves of the 64-bit product as one multiply (v_mad_u64_u32 on the
+// device rather than separate high and low multiplies).
+NZ_HD inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+  const uint64_t p = (uint64_t)a * (uint64_t)b;
+  hi = (uint32_t)(p >> 32);
+  lo = (uint32_t)p;
+}
+
 NZ_HD inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t hi0 = nz_umulhi(0xD2511F53u, c.x);
-    const uint32_t lo0 = 0xD2511F53u * c.x;
-    const uint32_t hi1 = nz_umulhi(0xCD9E8D57u, c.z);
-    const uint32_t lo1 = 0xCD9E8D57u * c.z;
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo(0xD2511F53u, c.x, hi0, lo0);
+    mulhilo(0xCD9E8D57u, c.z, hi1, lo1);
     c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;

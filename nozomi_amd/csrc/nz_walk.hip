@@ -133,7 +133,10 @@ struct StepParams {
 
 // Walk the spin held in slot i (fields strided by ld) from its next step to
 // at least P.target.  Every lane of the block calls it (inactive lanes too:
-// the per-step moment partials synchronise the block).
+// the per-step moment partials synchronise the block).  kTrack: the launch
+// records displacements or per-step moments, which keep the start position
+// in registers; the plain walk leaves those registers to the scan.
+template <bool kTrack>
 __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, int64_t ld,
                                           int64_t i, bool active, const StepParams P,
                                           float* smom) {
@@ -155,11 +158,11 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   int64_t t = active ? (int64_t)st.tstep[ii] : P.target;
   int u0[3] = {0, 0, 0};
   double p0[3] = {0.0, 0.0, 0.0};
-  if (P.stepmom) {
+  if (kTrack && P.stepmom) {
     for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x) smom[k] = 0.0f;
     __syncthreads();
   }
-  if (P.stepmom || P.rec) {
+  if (kTrack && (P.stepmom || P.rec)) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       u0[a] = active ? st.ucell0[a * ld + ii] : 0;
@@ -186,12 +189,12 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
         const float G = P.gph ? P.gph[t - P.base] : 0.0f;
         const float Ga = P.gph ? P.garea[t - P.base] : 0.0f;
         step_end(g, s, G, Ga);
-        if (P.rec) {  // displacement since the start, for the blocked phase contraction
+        if (kTrack && P.rec) {  // displacement since the start, for the blocked phase contraction
           const int64_t r = (t - P.mom_base) * 3;
 #pragma unroll
           for (int a = 0; a < 3; ++a) P.rec[(r + a) * ld + i] = (float)lane_disp(g, s, u0, p0, a);
         }
-        if (P.stepmom) {  // moment sums (sim3d_kernel.cu:191-237)
+        if (kTrack && P.stepmom) {  // moment sums (sim3d_kernel.cu:191-237)
           const int k = (int)(t - P.mom_base) * 6;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
@@ -205,7 +208,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
       }
     }
   }
-  if (P.stepmom) {  // block partials -> FP64 sums
+  if (kTrack && P.stepmom) {  // block partials -> FP64 sums
     __syncthreads();
     for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x)
       atomicAdd(&P.stepmom[(P.mom_base - P.base) * 6 + k], (double)smom[k]);
@@ -225,13 +228,14 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   if (st.nguard) st.nguard[i] += s.nguard;
 }
 
+template <bool kTrack>
 __global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A) {
-  __shared__ float smom[kMomChunk * 6];
+  __shared__ float smom[kTrack ? kMomChunk * 6 : 1];
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
   const int64_t i = blk * blockDim.x + threadIdx.x;
   const StepParams P{A.seed,  A.base,    A.target, A.limit, A.mom_base, A.dt,
                      A.gph,   A.garea,   A.stepmom, A.flags, A.rec};
-  walk_slot(A.g, A.st, A.n, i, i < A.n, P, smom);
+  walk_slot<kTrack>(A.g, A.st, A.n, i, i < A.n, P, smom);
 }
 
 // Absolute start positions (spin order) -> resident state (slot = spin).
@@ -734,7 +738,7 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR batch_walk_kernel(const SimD
   const int64_t target = min(d.step + d.k, d.nsteps);
   const StepParams P{d.seed, 0, target, d.nsteps, 0, d.dt, d.gph, d.garea, nullptr, d.flags,
                      nullptr};
-  walk_slot(d.g, st, ld, i, st.spin_id[i] >= 0, P, nullptr);
+  walk_slot<false>(d.g, st, ld, i, st.spin_id[i] >= 0, P, nullptr);
 }
 
 // After a launch: every active simulation has advanced to its target.
@@ -1065,7 +1069,10 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
       if (int rc = next_event_pair(w, &e0, &e1)) return rc;
       NZ_HIP(hipEventRecord(e0, s));
     }
-    hipLaunchKernelGGL(walk_kernel, dim3(blocks_for(w->n)), dim3(256), 0, s, a);
+    if (stepmom)
+      hipLaunchKernelGGL(walk_kernel<true>, dim3(blocks_for(w->n)), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL(walk_kernel<false>, dim3(blocks_for(w->n)), dim3(256), 0, s, a);
     NZ_HIP(hipGetLastError());
     if (w->timing) NZ_HIP(hipEventRecord(e1, s));
     target += chunk;
@@ -1102,7 +1109,7 @@ int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, flo
   a.flags = 0;
   a.xcd_chunk = w->xcd_chunk;
   a.rec = rec;
-  hipLaunchKernelGGL(walk_kernel, dim3(blocks_for(w->n)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(walk_kernel<true>, dim3(blocks_for(w->n)), dim3(256), 0, s, a);
   NZ_HIP(hipGetLastError());
   NZ_HIP(hipMemcpyAsync(slot_spin, w->buf[w->cur].spin_id, sizeof(int32_t) * w->n,
                         hipMemcpyDeviceToDevice, s));
