@@ -64,12 +64,27 @@ const char* nz_last_error_string(void);
  *   cell_size: target cell edge (um, 0 = 1.25); the grid tiles the box exactly
  *   margin   : per-cell list margin (um, 0 = 0.8); a sub-step that would
  *              leave the cell's box grown by the margin re-bins the spin
- * Synchronous (builds on the host, copies to the current device). */
+ * Synchronous: validates on the host, builds the lists, records and sort
+ * ranks on the current device (the reference builds its lists on the host). */
 int nz_substrate_create(const double* spheres_host, int64_t nspheres, int32_t nstructures,
                         const double* D_host, const double* T2_host, double Lx, double Ly,
                         double Lz, double cell_size, double margin, nz_substrate_t* out);
 int nz_substrate_destroy(nz_substrate_t sub);
 int nz_substrate_info(nz_substrate_t sub, nz_substrate_info_t* info);
+
+/* Copy a substrate's cell grid to host memory (inspection and tests; no
+ * counterpart in the reference, whose segment table is a host array,
+ * diffsim3d.py:166-174).  Each pointer may be NULL; sizes from
+ * nz_substrate_info with nloc = ncell + 2 per axis:
+ *   cell_start : int32 [nloc_x nloc_y nloc_z + 1], CSR list starts (cells
+ *                indexed (z nloc_y + y) nloc_x + x, halo cells included)
+ *   cell_rank  : int32 [ncell_x ncell_y ncell_z], sort rank of each in-box cell
+ *   cand       : float [8 (ncand_total + 1)], records (cx, cy, cz, sid) and
+ *                (r^2, (r-tol)^2, (r+tol)^2, sphere index bits) per entry,
+ *                coordinates relative to the cell's lower corner, then one
+ *                padding record.  Synchronous. */
+int nz_substrate_export_grid(nz_substrate_t sub, int32_t* cell_start, int32_t* cell_rank,
+                             float* cand);
 
 /* Rejection-seed n spins uniformly inside the target structures.
  * Replaces SimGeometry3D.gpu_seed + seedSpinsKernel (geometry.py:73-174,

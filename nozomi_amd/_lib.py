@@ -49,6 +49,7 @@ _SIGS = {
                                            ctypes.POINTER(_c_vp)]),
     "nz_substrate_destroy": (ctypes.c_int, [_c_vp]),
     "nz_substrate_info": (ctypes.c_int, [_c_vp, ctypes.POINTER(NzSubstrateInfo)]),
+    "nz_substrate_export_grid": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp]),
     "nz_seed": (ctypes.c_int, [_c_vp, ctypes.c_int64, _c_vp, ctypes.c_int32, ctypes.c_uint64,
                                ctypes.c_int32, _c_vp, _c_vp, _c_vp]),
     "nz_walker_create": (ctypes.c_int, [_c_vp, ctypes.c_int64, ctypes.c_uint32,

@@ -49,10 +49,14 @@ struct HostGrid {
   }
 };
 
+// Validation and the grid's scalar parameters, shared by the host builder
+// below and the device builder (nz_grid_gpu.hip): the cell counts and edges
+// in g, the FP64 edges hd and the list margin M (safety margin plus slack for
+// the tolerance band and the rounding of cell-relative coordinates).
 // Returns 0, or -1 with a message in err.
-inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, const double* D,
-                           const double* T2, const double L[3], double cell_size, double margin,
-                           HostGrid& out, std::string& err) {
+inline int grid_params(const double* spheres, int64_t S, int32_t nstruct, const double* D,
+                       const double* T2, const double L[3], double& cell_size, double& margin,
+                       Grid& g, double hd[3], double& M, std::string& err) {
   if (S < 0 || (S > 0 && !spheres)) { err = "bad sphere array"; return -1; }
   if (S > (int64_t)INT32_MAX) { err = "more than 2^31 spheres"; return -1; }
   if (nstruct < 0 || !D || !T2) { err = "bad structure arrays"; return -1; }
@@ -69,13 +73,10 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
   }
   if (cell_size <= 0) cell_size = 1.25;
   if (margin <= 0) margin = 0.8;
-  Grid& g = out.g;
   std::memset(&g, 0, sizeof(g));
-  std::memset(&out.info, 0, sizeof(out.info));
   g.nstruct = nstruct;
   g.nspheres = S;
   g.margin = (float)margin;
-  int64_t ncells = 1;
   for (int a = 0; a < 3; ++a) {
     g.ncell[a] = std::max(1, (int)std::lround(L[a] / cell_size));
     if (g.ncell[a] > 1000) { err = "more than 1000 cells per axis; raise cell_size"; return -1; }
@@ -85,12 +86,21 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
     g.L[a] = (float)L[a];
     g.hd[a] = L[a] / g.ncell[a];
     g.Ld[a] = L[a];
-    ncells *= g.nloc[a];
+    hd[a] = L[a] / g.ncell[a];
   }
-  const double hd[3] = {L[0] / g.ncell[0], L[1] / g.ncell[1], L[2] / g.ncell[2]};
-  // list margin: the safety margin plus slack for the tolerance band and for
-  // rounding of the cell-relative coordinates
-  const double M = margin + 4.0 * (double)kTol + 1e-5;
+  M = margin + 4.0 * (double)kTol + 1e-5;
+  return 0;
+}
+
+// Returns 0, or -1 with a message in err.
+inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, const double* D,
+                           const double* T2, const double L[3], double cell_size, double margin,
+                           HostGrid& out, std::string& err) {
+  Grid& g = out.g;
+  double hd[3], M;
+  if (grid_params(spheres, S, nstruct, D, T2, L, cell_size, margin, g, hd, M, err) != 0) return -1;
+  std::memset(&out.info, 0, sizeof(out.info));
+  const int64_t ncells = (int64_t)g.nloc[0] * g.nloc[1] * g.nloc[2];
   const double* cx = spheres;
   const double* cy = spheres + S;
   const double* cz = spheres + 2 * S;

@@ -73,6 +73,17 @@ int hip_fail(hipError_t e, const char* what);
 int signal_reduce(int64_t n, int64_t ld, const int32_t* valid, int32_t W, const float* mom,
                   const float* dirs, const float* scale, double* out, hipStream_t stream);
 
+// In-place exclusive scan of counts[0, m) (int32; the total must fit), with
+// a work array of scan_partial_len(m) ints; launches on `stream`.
+int64_t scan_partial_len(int64_t m);
+int exclusive_scan_i32(int32_t* counts, int64_t m, int32_t* partial, hipStream_t stream);
+
+// Build the cell grid of a substrate on the device (nz_grid_gpu.hip): the
+// same lists, records and sort ranks as build_grid_host (nz_grid_build.h).
+int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const double* D,
+                      const double* T2, const double L[3], double cell_size, double margin,
+                      Substrate* s);
+
 }  // namespace nz
 
 #define NZ_HIP(call)                                           \

@@ -17,7 +17,6 @@
 #include <string>
 #include <vector>
 
-#include "nz_grid_build.h"
 #include "nz_internal.h"
 
 namespace nz {
@@ -34,18 +33,6 @@ int fail(int code, const std::string& msg) {
 int hip_fail(hipError_t e, const char* what) {
   set_error(std::string(what) + ": " + hipGetErrorString(e));
   return (int)e;
-}
-
-template <class T>
-static int upload(Substrate* s, const std::vector<T>& v, const T** out) {
-  void* p = nullptr;
-  const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
-  NZ_HIP(hipMalloc(&p, bytes));
-  s->allocations.push_back(p);
-  if (!v.empty()) NZ_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  s->info.device_bytes += (int64_t)bytes;
-  *out = (const T*)p;
-  return 0;
 }
 
 static void release(Substrate* s) {
@@ -69,27 +56,31 @@ int nz_substrate_create(const double* spheres, int64_t S, int32_t nstruct, const
   if (!out) return fail(-1, "nz_substrate_create: out is NULL");
   *out = nullptr;
   const double L[3] = {Lx, Ly, Lz};
-  HostGrid hg;
-  std::string err;
-  if (build_grid_host(spheres, S, nstruct, D, T2, L, cell_size, margin, hg, err) != 0)
-    return fail(-1, "nz_substrate_create: " + err);
   Substrate* s = new Substrate();
   (void)hipGetDevice(&s->device);
-  s->g = hg.g;
-  s->info = hg.info;
-  s->info.device_bytes = 0;
-  Grid& g = s->g;
-  int rc = 0;
-  if ((rc = upload(s, hg.start, &g.cell_start)) || (rc = upload(s, hg.rank, &g.cell_rank)) ||
-      (rc = upload(s, hg.cand, &g.cand)) ||
-      (rc = upload(s, hg.sph, &g.sph)) ||
-      (rc = upload(s, hg.sid, &g.sph_sid)) || (rc = upload(s, hg.D, &g.D)) ||
-      (rc = upload(s, hg.T2, &g.T2)) || (rc = upload(s, hg.inv_T2, &g.inv_T2))) {
+  if (int rc = build_grid_device(spheres, S, nstruct, D, T2, L, cell_size, margin, s)) {
     release(s);
     delete s;
     return rc;
   }
   *out = (nz_substrate_t)s;
+  return 0;
+}
+
+int nz_substrate_export_grid(nz_substrate_t sub, int32_t* cell_start, int32_t* cell_rank,
+                             float* cand) {
+  if (!sub) return fail(-1, "nz_substrate_export_grid: NULL substrate");
+  const Substrate* s = (const Substrate*)sub;
+  const Grid& g = s->g;
+  const size_t ncells = (size_t)g.nloc[0] * g.nloc[1] * g.nloc[2];
+  const size_t nbox = (size_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
+  const size_t nrec = 2 * (size_t)s->info.ncand_total + 2;
+  if (cell_start)
+    NZ_HIP(hipMemcpy(cell_start, g.cell_start, sizeof(int32_t) * (ncells + 1),
+                     hipMemcpyDeviceToHost));
+  if (cell_rank)
+    NZ_HIP(hipMemcpy(cell_rank, g.cell_rank, sizeof(int32_t) * nbox, hipMemcpyDeviceToHost));
+  if (cand) NZ_HIP(hipMemcpy(cand, g.cand, sizeof(float4) * nrec, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -433,6 +433,21 @@ __global__ void __launch_bounds__(kScanThreads) tile_scan_kernel(int32_t* counts
   }
 }
 
+int64_t scan_partial_len(int64_t m) { return (m + kScanTile - 1) / kScanTile; }
+
+int exclusive_scan_i32(int32_t* counts, int64_t m, int32_t* partial, hipStream_t stream) {
+  const int64_t nb = scan_partial_len(m);
+  if (nb <= 0) return 0;
+  hipLaunchKernelGGL(tile_sum_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, counts, m,
+                     partial, (const int64_t*)nullptr);
+  hipLaunchKernelGGL(partial_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, partial, nb,
+                     (const int64_t*)nullptr);
+  hipLaunchKernelGGL(tile_scan_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, counts,
+                     m, (const int32_t*)partial, (const int64_t*)nullptr);
+  NZ_HIP(hipGetLastError());
+  return 0;
+}
+
 __global__ void __launch_bounds__(256) sort_scatter_kernel(const SlotState src, const SlotState dst,
                                                            int64_t n, const int32_t* keys,
                                                            const int32_t* offsets, int32_t* cursor) {

@@ -121,6 +121,21 @@ class DeviceSubstrate:
             "ncand_mean_box": inf.ncand_mean_box, "device_bytes": inf.device_bytes,
         }
 
+    def grid(self):
+        """The device-built cell grid as numpy arrays (nz_substrate_export_grid):
+        cell_start int32 [nloc^3 + 1], cell_rank int32 [ncell^3], cand float32
+        [2 (ncand_total + 1), 4] records."""
+        inf = self.info()
+        nloc = [n + 2 for n in inf["ncell"]]
+        start = np.empty(nloc[0] * nloc[1] * nloc[2] + 1, np.int32)
+        rank = np.empty(int(np.prod(inf["ncell"])), np.int32)
+        cand = np.empty((2 * (inf["ncand_total"] + 1), 4), np.float32)
+        check(lib().nz_substrate_export_grid(self.handle, start.ctypes.data_as(_c_vp),
+                                             rank.ctypes.data_as(_c_vp),
+                                             cand.ctypes.data_as(_c_vp)),
+              "nz_substrate_export_grid")
+        return start, rank, cand
+
     def close(self):
         if getattr(self, "handle", None):
             lib().nz_substrate_destroy(self.handle)

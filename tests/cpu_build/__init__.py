@@ -38,6 +38,7 @@ def lib():
                                  _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
         L.nzcpu_state_f64.argtypes = [_vp, _vp, _vp]
         L.nzcpu_info.argtypes = [_vp, _vp]
+        L.nzcpu_grid.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.nzcpu_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                     ctypes.c_uint32, ctypes.c_int, _vp]
         _lib = L
@@ -61,6 +62,17 @@ class CpuWalker:
     def __del__(self):
         if getattr(self, "h", None):
             lib().nzcpu_destroy(self.h)
+
+    def grid(self):
+        """The host builder's (cell_start, cell_rank, cand [2 (ntotal + 1), 4])."""
+        sizes = np.zeros(3, np.int64)
+        lib().nzcpu_grid(self.h, None, None, None, sizes.ctypes.data)
+        start = np.empty(sizes[0], np.int32)
+        rank = np.empty(sizes[1], np.int32)
+        cand = np.empty((sizes[2] // 4, 4), np.float32)
+        lib().nzcpu_grid(self.h, start.ctypes.data, rank.ctypes.data, cand.ctypes.data,
+                         sizes.ctypes.data)
+        return start, rank, cand
 
     def init(self, pos):
         self.pos_init = np.ascontiguousarray(pos, dtype=np.float32)

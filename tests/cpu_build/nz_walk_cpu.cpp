@@ -101,6 +101,18 @@ void nzcpu_walk(void* h, uint64_t seed, int64_t step0, int nsteps, double dt, co
   }
 }
 
+// The host builder's grid (same layout as nz_substrate_export_grid); sizes
+// from nzcpu_info: start [nloc^3 + 1], rank [ncell^3], cand [8 (ntotal + 1)].
+void nzcpu_grid(void* h, int32_t* start, int32_t* rank, float* cand, int64_t* sizes) {
+  const HostGrid& hg = ((CpuWalker*)h)->hg;
+  sizes[0] = (int64_t)hg.start.size();
+  sizes[1] = (int64_t)hg.rank.size();
+  sizes[2] = (int64_t)hg.cand.size() * 4;
+  if (start) std::memcpy(start, hg.start.data(), sizeof(int32_t) * hg.start.size());
+  if (rank) std::memcpy(rank, hg.rank.data(), sizeof(int32_t) * hg.rank.size());
+  if (cand) std::memcpy(cand, hg.cand.data(), sizeof(float4) * hg.cand.size());
+}
+
 // FP64 absolute positions (wrapped) and unwrapped displacements [3, N].
 void nzcpu_state_f64(void* h, double* pos, double* disp) {
   CpuWalker* w = (CpuWalker*)h;
