@@ -79,10 +79,10 @@ int nz_substrate_info(nz_substrate_t sub, nz_substrate_info_t* info);
  *   cell_start : int32 [nloc_x nloc_y nloc_z + 1], CSR list starts (cells
  *                indexed (z nloc_y + y) nloc_x + x, halo cells included)
  *   cell_rank  : int32 [ncell_x ncell_y ncell_z], sort rank of each in-box cell
- *   cand       : float [8 (ncand_total + 1)], records (cx, cy, cz, sid) and
- *                (r^2, (r-tol)^2, (r+tol)^2, sphere index bits) per entry,
- *                coordinates relative to the cell's lower corner, then one
- *                padding record.  Synchronous. */
+ *   cand       : float [8 (ncand_total + 4)], records (cx, cy, cz, (r+tol)^2)
+ *                and (r^2, (r-tol)^2, sid, sphere index bits) per entry,
+ *                coordinates relative to the cell's lower corner, then four
+ *                zero padding records.  Synchronous. */
 int nz_substrate_export_grid(nz_substrate_t sub, int32_t* cell_start, int32_t* cell_rank,
                              float* cand);
 

@@ -131,8 +131,9 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
                 " spheres; lower cell_size or margin";
           return -1;
         }
-      // one padding record after the last list (the scan loads one ahead)
-      out.cand.assign(2 * (size_t)cnt[ncells] + 2, make_float4(0, 0, 0, 0));
+      // padding records after the last list: the scan loads up to
+      // kCandPad records past the end of a list
+      out.cand.assign(2 * ((size_t)cnt[ncells] + kCandPad), make_float4(0, 0, 0, 0));
       fill.assign(cnt.begin(), cnt.end() - 1);
     }
     for (int64_t i = 0; i < S; ++i) {
@@ -147,9 +148,10 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
       const int32_t idx = (int32_t)i;
       float idx_bits;
       std::memcpy(&idx_bits, &idx, sizeof(idx_bits));
+      const float rout2 = (float)((ri + (double)kTol) * (ri + (double)kTol));
       const float4 bv = make_float4((float)(ri * ri),
                                     (float)((ri - (double)kTol) * (ri - (double)kTol)),
-                                    (float)((ri + (double)kTol) * (ri + (double)kTol)), idx_bits);
+                                    (float)sd[i], idx_bits);
       for (int z = lo[2]; z <= hi[2]; ++z)
         for (int y = lo[1]; y <= hi[1]; ++y)
           for (int x = lo[0]; x <= hi[0]; ++x) {
@@ -160,7 +162,7 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
               const int64_t k = fill[c]++;
               out.cand[2 * k] = make_float4((float)(cx[i] - cell_lo(0, x)),
                                             (float)(cy[i] - cell_lo(1, y)),
-                                            (float)(cz[i] - cell_lo(2, z)), (float)sd[i]);
+                                            (float)(cz[i] - cell_lo(2, z)), rout2);
               out.cand[2 * k + 1] = bv;
             }
           }

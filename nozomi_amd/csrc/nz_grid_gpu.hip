@@ -112,8 +112,8 @@ __global__ void grid_fill_kernel(const GridGeo G, int32_t* cursor, int32_t* ent)
 }
 
 // One thread per cell: sort the list by sphere index, then write the records
-// cand[2k] = (cx - lo_x, cy - lo_y, cz - lo_z, sid),
-// cand[2k+1] = (r^2, (r-tol)^2, (r+tol)^2, sphere index bits).
+// cand[2k] = (cx - lo_x, cy - lo_y, cz - lo_z, (r+tol)^2),
+// cand[2k+1] = (r^2, (r-tol)^2, sid, sphere index bits).
 __global__ void grid_records_kernel(const GridGeo G, const int32_t* start, int64_t ncells,
                                     int32_t* ent, float4* cand) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -140,9 +140,9 @@ __global__ void grid_records_kernel(const GridGeo G, const int32_t* start, int64
     __builtin_memcpy(&idx_bits, &i, sizeof(idx_bits));
     cand[2 * (int64_t)k] = make_float4((float)(G.sp[i] - lx), (float)(G.sp[G.S + i] - ly),
                                        (float)(G.sp[2 * G.S + i] - lz),
-                                       (float)G.sp[4 * G.S + i]);
+                                       (float)((ri + tol) * (ri + tol)));
     cand[2 * (int64_t)k + 1] = make_float4((float)(ri * ri), (float)((ri - tol) * (ri - tol)),
-                                           (float)((ri + tol) * (ri + tol)), idx_bits);
+                                           (float)G.sp[4 * G.S + i], idx_bits);
   }
 }
 
@@ -272,7 +272,7 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
   double4* sph = nullptr;
   if (int rc = dev_alloc(s, (size_t)ncells + 1, &start)) return rc;
   if (int rc = dev_alloc(s, (size_t)nbox, &rank)) return rc;
-  if (int rc = dev_alloc(s, 2 * (size_t)total + 2, &cand)) return rc;  // + a padding record
+  if (int rc = dev_alloc(s, 2 * ((size_t)total + kCandPad), &cand)) return rc;  // + padding
   if (int rc = dev_alloc(s, (size_t)S, &sph)) return rc;
   if (int rc = dev_alloc(s, (size_t)S, &sid)) return rc;
   if (int rc = tmp.alloc((size_t)total, &ent)) return rc;
@@ -282,7 +282,7 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
   NZ_HIP(hipMemcpyAsync(cursor, cnt, sizeof(int32_t) * (size_t)ncells, hipMemcpyDeviceToDevice,
                         st));
   hipLaunchKernelGGL(grid_zero_kernel, dim3(1), dim3(256), 0, st,
-                     (int32_t*)(cand + 2 * total), (int64_t)8);
+                     (int32_t*)(cand + 2 * total), (int64_t)(8 * kCandPad));
   if (S > 0) {
     hipLaunchKernelGGL(grid_fill_kernel, dim3(blocks256(S)), dim3(256), 0, st, G, cursor, ent);
     hipLaunchKernelGGL(grid_spheres_kernel, dim3(blocks256(S)), dim3(256), 0, st, G, sph, sid);

@@ -17,6 +17,8 @@ constexpr float kLarge = 1e6f;
 constexpr int kMaxIter = 100;
 // Longest candidate list of a cell (the FP32 scan keys a list position in 8 bits).
 constexpr int kMaxListLen = 255;
+// Zero records after the last candidate list (the scans load ahead).
+constexpr int kCandPad = 4;
 
 // Device view of a substrate.  The box [-L/2, L/2] per axis is tiled by
 // ncell[a] cells of edge h[a]; one halo cell on each side (local index 0 and
@@ -37,15 +39,15 @@ struct Grid {
   int64_t nspheres;
   // CSR candidate lists over the nloc^3 local cells; entry e is the 32-byte
   // record cand[2e] = a, cand[2e+1] = b with cell-relative coordinates:
-  //   a = (cx - lo_x, cy - lo_y, cz - lo_z, sid)
-  //   b = (r^2, (r-tol)^2, (r+tol)^2, sphere index as int bits)
+  //   a = (cx - lo_x, cy - lo_y, cz - lo_z, (r+tol)^2)
+  //   b = (r^2, (r-tol)^2, sid, sphere index as int bits)
   // A list holds at most 255 entries.
   const int32_t* cell_start;  // [ncells + 1]
   // sort position of each in-box cell (index (z ncell_y + y) ncell_x + x):
   // cells grouped in 4x4x4 regions, ordered by list length within a region,
   // so a wavefront's spins tend to scan lists of similar length
   const int32_t* cell_rank;
-  const float4* cand;         // [2 * entries + 2]: a padding record at the end
+  const float4* cand;         // [2 * (entries + kCandPad)]: padding records at the end
   // every sphere, absolute, FP64: the exact re-solves and the rare sub-step
   // that leaves the grid's lists
   const double4* sph;         // (cx, cy, cz, r)

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) seed_kernel(const SeedArgs A) {
       const float4 B = g.cand[2 * (int64_t)q + 1];
       const float dx = p[0] - S.x, dy = p[1] - S.y, dz = p[2] - S.z;
       if (dx * dx + dy * dy + dz * dz < B.x) {  // sim3d_kernel.cu:299
-        const int sid = (int)S.w;
+        const int sid = (int)B.z;
         if (is_target(A, sid)) min_t = min(min_t, sid);
         else min_n = min(min_n, sid);
       }

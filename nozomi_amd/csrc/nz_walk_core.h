@@ -121,6 +121,10 @@ struct ScanAcc {
 struct ScanIn {
   float p[3], j[3];
   float a4, inv_a, sid;  // 4 |j|^2, 1 / |j|^2, structure id assumed for the scan
+  // reach bound (scan_list): with l the sub-step's segment length plus slack,
+  // a sphere whose (r+tol)-ball the segment cannot touch has
+  // dd > (r+tol)^2 reach1 + reach0
+  float reach1, reach0;
 };
 
 // Accumulators of the FP32 candidate loop.  A decision and the list position
@@ -184,9 +188,9 @@ NZ_HD inline void visit_exact(const Grid& g, const double p[3], const double j[3
 }
 
 // One candidate sphere: the structure test (sim3d_kernel.cu:380-398) and the
-// collision test (:414-455) fused, without branches.  A = (cx, cy, cz, sid)
-// relative to the lane's cell corner, B = (r^2, (r-tol)^2, (r+tol)^2, sphere
-// index bits).  One quadratic a f^2 + b f + c (c = dd - (r -/+ tol)^2) serves
+// collision test (:414-455) fused, without branches.  A = (cx, cy, cz,
+// (r+tol)^2) relative to the lane's cell corner, B = (r^2, (r-tol)^2, sid,
+// sphere index bits).  One quadratic a f^2 + b f + c (c = dd - (r -/+ tol)^2) serves
 // both reference roots (findStepSizeToSphereInsideTol / ...OutsideTol,
 // :77-160), whose branch rules reduce to: with q = -(b - sqrt(disc))/2 for
 // b <= 0 (q >= 0, far root q/a, near root c/q) and q = -(b + sqrt(disc))/2
@@ -201,11 +205,11 @@ NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, uint32
   const float hb = fmaf(in.j[2], dz, fmaf(in.j[1], dy, in.j[0] * dx));
   const float b = hb + hb;
   const bool inside = din < 0.0f;                              // isInsideSphere < 0
-  const float R2 = inside ? B.y : B.z;
+  const float R2 = inside ? B.y : A.w;
   const float c = dd - R2;
   const float b2 = b * b;
   const float disc = fmaf(-in.a4, c, b2);
-  const bool relevant = A.w <= in.sid;                         // :418-419
+  const bool relevant = B.z <= in.sid;                         // :418-419
   // FP32 cannot decide this candidate: it is re-solved in FP64 after the loop
   // (bitwise bool algebra keeps the masks in scalar registers)
   const bool g_in = fabsf(din) < kGuardInside * (dd + B.x);
@@ -213,8 +217,8 @@ NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, uint32
   const bool guard = g_in | (relevant & g_disc);
   k.gcnt += guard ? 1 : 0;
   k.gpos = guard ? pos1 : k.gpos;
-  const bool take_sid = inside & !guard & (A.w < k.sid_new);  // :384-396
-  k.sid_new = take_sid ? A.w : k.sid_new;
+  const bool take_sid = inside & !guard & (B.z < k.sid_new);  // :384-396
+  k.sid_new = take_sid ? B.z : k.sid_new;
   const float sq = nz_sqrtf(fmaxf(disc, 0.0f));
   const bool away = b > 0.0f;
   const float q = -0.5f * (b + (away ? sq : -sq));
@@ -226,7 +230,7 @@ NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, uint32
   const uint32_t key_in = (f2u(f_in) & kKeyMask) | pos1;
   const uint32_t key_out = (f2u(f_out) & kKeyMask) | pos1;
   const bool use_in = relevant & inside & !guard;                     // :428-440
-  const bool use_out = !inside & (A.w < in.sid) & !guard;             // :441-454
+  const bool use_out = !inside & (B.z < in.sid) & !guard;             // :441-454
   k.kin = (use_in & (key_in > k.kin)) ? key_in : k.kin;
   k.kout = (use_out & (key_out < k.kout)) ? key_out : k.kout;
 }
@@ -285,12 +289,10 @@ NZ_HD inline int cand_sphere(const Grid& g, int e) {
   return (int)f2u(g.cand[2 * (int64_t)e + 1].w);
 }
 
-// The candidate loop over the list [cs, ce), with the next record's loads
-// issued before the current one is evaluated.  Reading one record past the
-// list is in bounds: the record array ends with a padding record.
-NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, ScanKeys& k) {
-  const int n = ce - cs;
-  const float4* rec = g.cand + 2 * (int64_t)cs;
+// Every candidate of the list through the full visit, with the next record's
+// loads issued before the current one is evaluated.  Loads past the list
+// stay in bounds: the record array ends with padding records.
+NZ_HD inline void scan_all(const float4* rec, int n, const ScanIn& in, ScanKeys& k) {
   float4 A0 = rec[0];
   float4 B0 = rec[1];
   int e = 0;
@@ -304,6 +306,74 @@ NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, Sca
     rec += 4;
   }
   if (e < n) visit(in, A0, B0, (uint32_t)(e + 1), k);
+}
+
+// Can the sub-step's segment touch the candidate's (r+tol)-ball?  A sphere it
+// cannot touch neither contains the spin (dd > (r+tol)^2 > r^2), nor lies in
+// the inside test's guard band, nor can be entered before jr: it cannot
+// change the keys.  The bound avoids r: 2R <= R^2 + 1 gives (R + l)^2 <=
+// R^2 (1 + l) + l + l^2, so dd above that is out of reach.
+NZ_HD inline bool reachable(const ScanIn& in, const float4 A) {
+  const float dx = in.p[0] - A.x, dy = in.p[1] - A.y, dz = in.p[2] - A.z;
+  const float dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  return dd <= fmaf(A.w, in.reach1, in.reach0);
+}
+
+// Lowest set position of the two masks, cleared; `none` when both are empty.
+NZ_HD inline int pop_flag(uint32_t& m0, uint32_t& m1, int none) {
+  if (m0 != 0u) {
+    const int q = __builtin_ctz(m0);
+    m0 &= m0 - 1u;
+    return q;
+  }
+  if (m1 != 0u) {
+    const int q = 32 + __builtin_ctz(m1);
+    m1 &= m1 - 1u;
+    return q;
+  }
+  return none;
+}
+
+// The candidate scan over the list [cs, ce).  A reach pass over every
+// candidate (its first float4 only) marks the spheres the segment can touch
+// as bits of two 32-bit masks (the loop position is the same on every lane);
+// only those get the full visit, a few per list against ~15 candidates.
+// The keys equal those of visiting every candidate, up to the guard count of
+// out-of-reach spheres (whose FP64 re-solve could not change the sub-step).
+// Lists longer than 64 take the full visit for every candidate.
+NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, ScanKeys& k) {
+  const int n = ce - cs;
+  const float4* rec = g.cand + 2 * (int64_t)cs;
+  if (n > 64) {
+    scan_all(rec, n, in, k);
+    return;
+  }
+  uint32_t m0 = 0u, m1 = 0u;
+  float4 A0 = rec[0], A1 = rec[2];
+  for (int e = 0; e < n; e += 2) {  // two candidates per trip, loaded one trip ahead
+    const float4 N0 = rec[4], N1 = rec[6];
+    const uint32_t b = (reachable(in, A0) ? 1u : 0u) | ((e + 1 < n && reachable(in, A1)) ? 2u : 0u);
+    if (e < 32) m0 |= b << e;
+    else m1 |= b << (e - 32);
+    A0 = N0;
+    A1 = N1;
+    rec += 4;
+  }
+  // the marked records, each loaded while the previous one is visited; the
+  // position n (none left) loads the record after the list, not visited
+  rec = g.cand + 2 * (int64_t)cs;
+  int q = pop_flag(m0, m1, n);
+  float4 A = rec[2 * q];
+  float4 B = rec[2 * q + 1];
+  while (q < n) {
+    const int qn = pop_flag(m0, m1, n);
+    const float4 An = rec[2 * qn];
+    const float4 Bn = rec[2 * qn + 1];
+    visit(in, A, B, (uint32_t)(q + 1), k);
+    q = qn;
+    A = An;
+    B = Bn;
+  }
 }
 
 // The keys of a finished scan as the FP64 code's accumulator.
@@ -504,6 +574,11 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
       in.a4 = 4.0f * a;
       in.inv_a = 1.0f / a;
       in.sid = s.sid;
+      // segment length with slack (relative 1e-3, absolute 1e-4 um) for the
+      // FP32 round-off of the reach bound
+      const float ell = sqrtf(a) * (float)q.jr * 1.001f + 1e-4f;
+      in.reach1 = 1.0f + ell;
+      in.reach0 = ell + ell * ell;
       ScanKeys k;
       keys_reset(g, k);
       scan_list(g, q.cs, q.ce, in, k);
