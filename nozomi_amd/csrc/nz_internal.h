@@ -18,7 +18,7 @@ constexpr int kMaxIter = 100;
 // Longest candidate list of a cell (the FP32 scan keys a list position in 8 bits).
 constexpr int kMaxListLen = 255;
 // Zero records after the last candidate list (the scans load ahead).
-constexpr int kCandPad = 4;
+constexpr int kCandPad = 8;
 
 // Device view of a substrate.  The box [-L/2, L/2] per axis is tiled by
 // ncell[a] cells of edge h[a]; one halo cell on each side (local index 0 and

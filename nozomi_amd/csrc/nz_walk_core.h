@@ -335,8 +335,9 @@ NZ_HD inline int pop_flag(uint32_t& m0, uint32_t& m1, int none) {
 }
 
 // The candidate scan over the list [cs, ce).  A reach pass over every
-// candidate (its first float4 only) marks the spheres the segment can touch
-// as bits of two 32-bit masks (the loop position is the same on every lane);
+// candidate (its first float4 only; four independent tests per trip, the
+// next four records in flight) marks the spheres the segment can touch as
+// bits of two 32-bit masks (the loop position is the same on every lane);
 // only those get the full visit, a few per list against ~15 candidates.
 // The keys equal those of visiting every candidate, up to the guard count of
 // out-of-reach spheres (whose FP64 re-solve could not change the sub-step).
@@ -349,15 +350,19 @@ NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, Sca
     return;
   }
   uint32_t m0 = 0u, m1 = 0u;
-  float4 A0 = rec[0], A1 = rec[2];
-  for (int e = 0; e < n; e += 2) {  // two candidates per trip, loaded one trip ahead
-    const float4 N0 = rec[4], N1 = rec[6];
-    const uint32_t b = (reachable(in, A0) ? 1u : 0u) | ((e + 1 < n && reachable(in, A1)) ? 2u : 0u);
+  float4 A0 = rec[0], A1 = rec[2], A2 = rec[4], A3 = rec[6];
+  for (int e = 0; e < n; e += 4) {  // four candidates per trip, loaded one trip ahead
+    const float4 N0 = rec[8], N1 = rec[10], N2 = rec[12], N3 = rec[14];
+    const uint32_t b = (reachable(in, A0) ? 1u : 0u) | ((e + 1 < n && reachable(in, A1)) ? 2u : 0u) |
+                       ((e + 2 < n && reachable(in, A2)) ? 4u : 0u) |
+                       ((e + 3 < n && reachable(in, A3)) ? 8u : 0u);
     if (e < 32) m0 |= b << e;
     else m1 |= b << (e - 32);
     A0 = N0;
     A1 = N1;
-    rec += 4;
+    A2 = N2;
+    A3 = N3;
+    rec += 8;
   }
   // the marked records, each loaded while the previous one is visited; the
   // position n (none left) loads the record after the list, not visited
