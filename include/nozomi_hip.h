@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define NZ_ABI_VERSION 1
+#define NZ_ABI_VERSION 2
 
 typedef struct nz_substrate* nz_substrate_t;
 
@@ -48,6 +48,8 @@ typedef struct nz_substrate_info_t {
   int32_t ncand_max;       /* largest list                               */
   double ncand_mean_box;   /* mean list length over the in-box cells     */
   int64_t device_bytes;    /* device memory held by the handle           */
+  int64_t ncand_records;   /* records in the grid's record array, the
+                              padding records after the last list included */
 } nz_substrate_info_t;
 
 int nz_abi_version(void);
@@ -79,10 +81,11 @@ int nz_substrate_info(nz_substrate_t sub, nz_substrate_info_t* info);
  *   cell_start : int32 [nloc_x nloc_y nloc_z + 1], CSR list starts (cells
  *                indexed (z nloc_y + y) nloc_x + x, halo cells included)
  *   cell_rank  : int32 [ncell_x ncell_y ncell_z], sort rank of each in-box cell
- *   cand       : float [8 (ncand_total + 4)], records (cx, cy, cz, (r+tol)^2)
+ *   cand       : float [8 ncand_records], records (cx, cy, cz, (r+tol)^2)
  *                and (r^2, (r-tol)^2, sid, sphere index bits) per entry,
- *                coordinates relative to the cell's lower corner, then four
- *                zero padding records.  Synchronous. */
+ *                coordinates relative to the cell's lower corner, then
+ *                ncand_records - ncand_total zero padding records.
+ *                Synchronous. */
 int nz_substrate_export_grid(nz_substrate_t sub, int32_t* cell_start, int32_t* cell_rank,
                              float* cand);
 

@@ -9,7 +9,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libnozomi_hip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 NZ_WALK_T2 = 1
 NZ_WALKER_COUNTERS = 1
@@ -29,6 +29,7 @@ class NzSubstrateInfo(ctypes.Structure):
         ("ncand_max", ctypes.c_int32),
         ("ncand_mean_box", ctypes.c_double),
         ("device_bytes", ctypes.c_int64),
+        ("ncand_records", ctypes.c_int64),
     ]
 
 

@@ -214,6 +214,7 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
   }
   inf.margin = margin;
   inf.ncand_total = cnt[ncells];
+  inf.ncand_records = cnt[ncells] + kCandPad;
   int32_t mx = 0;
   double box_sum = 0;
   for (int z = 0; z < g.nloc[2]; ++z)

@@ -325,6 +325,7 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
   }
   inf.margin = margin;
   inf.ncand_total = total;
+  inf.ncand_records = total + kCandPad;
   inf.ncand_max = (int32_t)hs[0];
   inf.ncand_mean_box = (double)hs[2] / (double)nbox;
   return 0;

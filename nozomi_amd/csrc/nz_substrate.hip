@@ -74,7 +74,7 @@ int nz_substrate_export_grid(nz_substrate_t sub, int32_t* cell_start, int32_t* c
   const Grid& g = s->g;
   const size_t ncells = (size_t)g.nloc[0] * g.nloc[1] * g.nloc[2];
   const size_t nbox = (size_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
-  const size_t nrec = 2 * ((size_t)s->info.ncand_total + kCandPad);
+  const size_t nrec = 2 * (size_t)s->info.ncand_records;
   if (cell_start)
     NZ_HIP(hipMemcpy(cell_start, g.cell_start, sizeof(int32_t) * (ncells + 1),
                      hipMemcpyDeviceToHost));

@@ -119,17 +119,18 @@ class DeviceSubstrate:
             "ncell": tuple(inf.ncell), "cell": tuple(inf.cell), "margin": inf.margin,
             "ncand_total": inf.ncand_total, "ncand_max": inf.ncand_max,
             "ncand_mean_box": inf.ncand_mean_box, "device_bytes": inf.device_bytes,
+            "ncand_records": inf.ncand_records,
         }
 
     def grid(self):
         """The device-built cell grid as numpy arrays (nz_substrate_export_grid):
         cell_start int32 [nloc^3 + 1], cell_rank int32 [ncell^3], cand float32
-        [2 (ncand_total + 4), 4] records (four padding records at the end)."""
+        [2 ncand_records, 4] records (padding records at the end)."""
         inf = self.info()
         nloc = [n + 2 for n in inf["ncell"]]
         start = np.empty(nloc[0] * nloc[1] * nloc[2] + 1, np.int32)
         rank = np.empty(int(np.prod(inf["ncell"])), np.int32)
-        cand = np.empty((2 * (inf["ncand_total"] + 4), 4), np.float32)
+        cand = np.empty((2 * inf["ncand_records"], 4), np.float32)
         check(lib().nz_substrate_export_grid(self.handle, start.ctypes.data_as(_c_vp),
                                              rank.ctypes.data_as(_c_vp),
                                              cand.ctypes.data_as(_c_vp)),

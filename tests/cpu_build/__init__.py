@@ -64,7 +64,8 @@ class CpuWalker:
             lib().nzcpu_destroy(self.h)
 
     def grid(self):
-        """The host builder's (cell_start, cell_rank, cand [2 (ntotal + 4), 4])."""
+        """The host builder's (cell_start, cell_rank, cand [2 records, 4]), padding
+        records included."""
         sizes = np.zeros(3, np.int64)
         lib().nzcpu_grid(self.h, None, None, None, sizes.ctypes.data)
         start = np.empty(sizes[0], np.int32)
