@@ -339,8 +339,10 @@ NZ_HD inline int pop_flag(uint32_t& m0, uint32_t& m1, int none) {
 // next four records in flight) marks the spheres the segment can touch as
 // bits of two 32-bit masks (the loop position is the same on every lane);
 // only those get the full visit, a few per list against ~15 candidates.
-// The keys equal those of visiting every candidate, up to the guard count of
-// out-of-reach spheres (whose FP64 re-solve could not change the sub-step).
+// Against visiting every candidate the exit key and the structure id are
+// equal; the entry key can only differ by entries beyond jr (the skipped
+// obstacles'), which cannot decide the sub-step, and the guard count by
+// out-of-reach spheres, whose FP64 re-solve could not change it either.
 // Lists longer than 64 take the full visit for every candidate.
 NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, ScanKeys& k) {
   const int n = ce - cs;

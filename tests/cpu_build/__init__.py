@@ -39,6 +39,7 @@ def lib():
         L.nzcpu_state_f64.argtypes = [_vp, _vp, _vp]
         L.nzcpu_info.argtypes = [_vp, _vp]
         L.nzcpu_grid.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.nzcpu_scan_compare.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
         L.nzcpu_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                     ctypes.c_uint32, ctypes.c_int, _vp]
         _lib = L
@@ -62,6 +63,20 @@ class CpuWalker:
     def __del__(self):
         if getattr(self, "h", None):
             lib().nzcpu_destroy(self.h)
+
+    def scan_compare(self, pos, jump, jr, sid):
+        """Per probe, which keys of the two-phase scan differ from visiting
+        every candidate: int32 [n, 4] (kin, kout, sid_new, guards beyond the
+        allowed difference)."""
+        n = pos.shape[1]
+        pos = np.ascontiguousarray(pos, np.float64)
+        jump = np.ascontiguousarray(jump, np.float64)
+        jr = np.ascontiguousarray(jr, np.float64)
+        sid = np.ascontiguousarray(sid, np.float32)
+        out = np.zeros((n, 4), np.int32)
+        lib().nzcpu_scan_compare(self.h, n, pos.ctypes.data, jump.ctypes.data, jr.ctypes.data,
+                                 sid.ctypes.data, out.ctypes.data)
+        return out
 
     def grid(self):
         """The host builder's (cell_start, cell_rank, cand [2 records, 4]), padding

@@ -43,6 +43,58 @@ void* nzcpu_create(const double* spheres, int64_t S, int32_t nstruct, const doub
 
 void nzcpu_destroy(void* h) { delete (CpuWalker*)h; }
 
+// The two-phase candidate scan against the visit of every candidate, for n
+// probes: positions pos [3, n] (absolute), jump vectors jump [3, n], remaining
+// fractions jr [n], assumed structure ids sid [n].  Each probe scans the list
+// of the cell holding its position.  out[4 i + k] = 1 where
+//   k = 0: the exit keys (kin) differ;
+//   k = 1: the entry keys (kout) differ although one of them is a root that
+//          can decide the sub-step (not beyond jr 1.0001 + 1e-6, the bound of
+//          the FP64 re-solve); entries beyond it cannot, and the two-phase scan
+//          skips the obstacles they belong to;
+//   k = 2: the structure ids differ;
+//   k = 3: the guards disagree beyond the allowed difference (the two-phase
+//          scan skips out-of-reach guards: it may count fewer, never more, and
+//          a single guard must be the same one).
+void nzcpu_scan_compare(void* h, int n, const double* pos, const double* jump, const double* jr,
+                        const float* sid, int32_t* out) {
+  const Grid& g = ((CpuWalker*)h)->hg.g;
+  for (int i = 0; i < n; ++i) {
+    Lane s;
+    const float p[3] = {(float)pos[i], (float)pos[n + i], (float)pos[2 * n + i]};
+    lane_init(g, p, s);
+    StepState q;
+    q.jr = jr[i];
+    for (int a = 0; a < 3; ++a) q.j[a] = jump[a * n + i];
+    const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
+    const int cs = g.cell_start[cid], ce = g.cell_start[cid + 1];
+    ScanIn in;
+    for (int a = 0; a < 3; ++a) {
+      in.p[a] = (float)s.p[a];
+      in.j[a] = (float)q.j[a];
+    }
+    const float a2 = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
+    in.a4 = 4.0f * a2;
+    in.inv_a = 1.0f / a2;
+    in.sid = sid[i];
+    const float ell = sqrtf(a2) * (float)q.jr * 1.001f + 1e-4f;
+    in.reach1 = 1.0f + ell;
+    in.reach0 = ell + ell * ell;
+    ScanKeys k2, k1;
+    keys_reset(g, k2);
+    keys_reset(g, k1);
+    scan_list(g, cs, ce, in, k2);
+    scan_all(g.cand + 2 * (int64_t)cs, ce - cs, in, k1);
+    out[4 * i + 0] = k2.kin != k1.kin;
+    const float lim = (float)q.jr * 1.0001f + 1e-6f;
+    const float f1 = (k1.kout != 0xffffffffu) ? u2f(k1.kout & kKeyMask) : kLarge;
+    const float f2 = (k2.kout != 0xffffffffu) ? u2f(k2.kout & kKeyMask) : kLarge;
+    out[4 * i + 1] = (k2.kout != k1.kout) && (f1 <= lim || f2 <= lim);
+    out[4 * i + 2] = k2.sid_new != k1.sid_new;
+    out[4 * i + 3] = (k2.gcnt > k1.gcnt) || (k1.gcnt == 1 && k2.gcnt == 1 && k1.gpos != k2.gpos);
+  }
+}
+
 // the walk's normals (host build of nz::normals3), n draws from draw0
 void nzcpu_normals(uint64_t seed, uint32_t stream, uint32_t spin, uint32_t step, uint32_t draw0,
                    int n, float* out) {

@@ -96,3 +96,37 @@ def test_oracle_is_chaotic(c0):
     assert many.sum() > 20
     assert np.mean(err[many] > 1e-5) > 0.5        # collision-rich spins diverge
     assert np.mean(err[res[0][1] <= 10] < 1e-7) > 0.99   # collision-poor spins do not
+
+
+def test_two_phase_scan_equals_full_scan(c0):
+    """The reach pass may only skip spheres that cannot change the sub-step:
+    against visiting every candidate of the list, the exit key and the
+    structure id are equal, the entry key is equal whenever either is a root
+    the sub-step can take (entries beyond jr belong to skipped obstacles), and
+    guards differ only by skipped out-of-reach ones.  Uniform probes and
+    probes on and near sphere surfaces (inside and outside the tolerance
+    shells), segments from 1e-4 um to the list margin, every remaining
+    fraction, every assumed structure id."""
+    g, _ = c0
+    cw = CpuWalker(g, cell=1.25, margin=0.8)
+    rng = np.random.default_rng(7)
+    L = g.Lx
+    sp = g.spheres
+    n_u, n_s = 20000, 60000
+    pos_u = rng.uniform(-L / 2, L / 2, size=(3, n_u))
+    k = rng.integers(0, sp.shape[1], n_s)
+    d = rng.normal(size=(3, n_s))
+    d /= np.linalg.norm(d, axis=0)
+    shell = sp[3, k] + rng.choice([-1.0, 1.0], n_s) * 10.0 ** rng.uniform(-7, -3, n_s)
+    pos_s = sp[:3, k] + d * shell
+    pos = np.concatenate([pos_u, pos_s], axis=1)
+    pos = pos - L * np.round(pos / L)                     # inside the box
+    pos = np.clip(pos, -L / 2 + 1e-6, L / 2 - 1e-6)
+    n = pos.shape[1]
+    jd = rng.normal(size=(3, n))
+    jd /= np.linalg.norm(jd, axis=0)
+    jump = jd * 10.0 ** rng.uniform(-4, np.log10(0.7), n)
+    jr = 10.0 ** rng.uniform(-4, 0, n)
+    sid = rng.integers(0, g.nstructures + 1, n).astype(np.float32)
+    out = cw.scan_compare(pos, jump, jr, sid)
+    assert out.sum(0).tolist() == [0, 0, 0, 0]
