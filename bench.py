@@ -226,37 +226,35 @@ def precision_form(args, geom, sub):
            "workload": "200-axon substrate, 4e6 spins, 1024 PGSE/OGSE waveforms (25-150 Hz, "
                        "|G| <= 300 mT/m, seeded directions), steps as given"}
     sig = {}
-    for mode in ("mfma", "fp32"):
-        pr = PrecisionRun(sub, x0, args.seed, g, u, dt, K=K, mode=mode)
+    # the contraction kernels timed alone (no overlap), then the whole form
+    # with the contraction of block b overlapping the walk of block b + 1
+    for mode, overlap in (("mfma", False), ("fp32", False), ("mfma", True)):
+        pr = PrecisionRun(sub, x0, args.seed, g, u, dt, K=K, mode=mode, overlap=overlap)
         pr.reset()
         pr.block(0)  # warm-up block (kernels, sort)
         pr.reset()
         torch.cuda.synchronize()
-        ev = []
-        stream = torch.cuda.current_stream()
+        if not overlap:
+            pr.gemm_events = []
         t0 = time.perf_counter()
         for b0 in range(0, T, K):
-            # time the contraction alone: events around nz_phase_gemm via the hook
-            holder = {}
-
-            def hook(_t0, _k, _rec, _ss, holder=holder):
-                e = torch.cuda.Event(enable_timing=True)
-                e.record(stream)
-                holder["a"] = e
-            pr.block(b0, record_hook=hook)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e1.record(stream)
-            ev.append((holder["a"], e1))
-        sig[mode] = pr.signals()
+            pr.block(b0)
+        s = pr.signals()
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        gemm_ms = [a.elapsed_time(b) for a, b in ev]
-        useful = 6.0 * W * N * K  # FLOP per full block
-        full = [m for m, (b0) in zip(gemm_ms, range(0, T, K)) if b0 + K <= T]
-        avg = float(np.mean(full))
-        res[mode] = {"seconds": el, "spin_steps_per_s": N * T / el,
-                     "contraction_ms_per_block": avg,
-                     "contraction_useful_tflops": useful / (avg * 1e-3) / 1e12}
+        if overlap:
+            res[mode]["overlapped_seconds"] = el
+            res[mode]["overlapped_spin_steps_per_s"] = N * T / el
+            sig[mode + "_overlap"] = s
+        else:
+            sig[mode] = s
+            useful = 6.0 * W * N * K  # FLOP per full block
+            full = [a.elapsed_time(b) for (a, b), b0 in zip(pr.gemm_events, range(0, T, K))
+                    if b0 + K <= T]
+            avg = float(np.mean(full))
+            res[mode] = {"seconds": el, "spin_steps_per_s": N * T / el,
+                         "contraction_ms_per_block": avg,
+                         "contraction_useful_tflops": useful / (avg * 1e-3) / 1e12}
         pr.close()
         del pr
         torch.cuda.empty_cache()
@@ -266,6 +264,7 @@ def precision_form(args, geom, sub):
     m["roofline"] = {"bound": "mfma", "achieved": m["contraction_executed_tflops"], "peak": peak,
                      "unit": "TFLOP/s", "frac": m["contraction_executed_tflops"] / peak}
     res["max_signal_diff_mfma_vs_fp32"] = float(np.abs(sig["mfma"] - sig["fp32"]).max())
+    res["max_signal_diff_overlap_vs_serial"] = float(np.abs(sig["mfma_overlap"] - sig["mfma"]).max())
     return res
 
 

@@ -58,9 +58,14 @@ def precision_waveforms(W, T, dt, seed=20261203, gmax=300.0):
 
 class PrecisionRun:
     """Walk N spins (resident walker) for T steps and contract every K-step
-    block with the W waveforms.  mode: "mfma" (split BF16) or "fp32"."""
+    block with the W waveforms.  mode: "mfma" (split BF16) or "fp32".
 
-    def __init__(self, sub, x0, seed, g, u, dt, K=64, mode="mfma"):
+    overlap: the walk of block b+1 (vector ALUs) runs on the walker's stream
+    while the contraction of block b (matrix cores, or the FP32 kernel) runs on
+    a second stream; the displacement record, its slot order and the waveform
+    block are double-buffered and events order each buffer's reuse."""
+
+    def __init__(self, sub, x0, seed, g, u, dt, K=64, mode="mfma", overlap=True):
         self.device = sub.device
         self.N = x0.shape[1]
         self.W, self.T = g.shape
@@ -76,45 +81,88 @@ class PrecisionRun:
         self.gsum = torch.from_numpy(g.astype(np.float64).sum(1).astype(np.float32)).to(d)
         self.walker = rt.Walker(sub, self.N, sort_interval=self.K)
         self.phase = torch.empty((self.N, self.W), dtype=torch.float32, device=d)
-        self.rec = torch.zeros((3 * self.K, self.N), dtype=torch.float32, device=d)
-        self.slot_spin = torch.empty(self.N, dtype=torch.int32, device=d)
-        self.bt_hi = torch.empty((self.W, 3 * self.K), dtype=torch.int16, device=d)
-        self.bt_lo = torch.empty_like(self.bt_hi)
-        self.bt_f32 = torch.empty((self.W, 3 * self.K), dtype=torch.float32, device=d) \
-            if self.mode == 1 else None
+        nb = 2 if overlap else 1
+        self.rec = [torch.zeros((3 * self.K, self.N), dtype=torch.float32, device=d)
+                    for _ in range(nb)]
+        self.slot_spin = [torch.empty(self.N, dtype=torch.int32, device=d) for _ in range(nb)]
+        self.bt_hi = [torch.empty((self.W, 3 * self.K), dtype=torch.int16, device=d)
+                      for _ in range(nb)]
+        self.bt_lo = [torch.empty_like(b) for b in self.bt_hi]
+        self.bt_f32 = [torch.empty((self.W, 3 * self.K), dtype=torch.float32, device=d)
+                       if self.mode == 1 else None for _ in range(nb)]
         self.signals_out = torch.empty((self.W, 2), dtype=torch.float64, device=d)
+        self.walk_stream = torch.cuda.current_stream(d)
+        self.gemm_stream = torch.cuda.Stream(d) if overlap else self.walk_stream
+        self.walk_done = [torch.cuda.Event() for _ in range(nb)]
+        self.gemm_done = [torch.cuda.Event() for _ in range(nb)]
+        self._used = [False] * nb
+        self._nb = nb
+        self._idx = 0
+        self.gemm_events = None  # list of (start, end) timing events when set to []
 
     def reset(self):
         L = lib()
-        with torch.cuda.device(self.device):
+        with torch.cuda.device(self.device), torch.cuda.stream(self.walk_stream):
+            # contractions still in flight update the phases being reset
+            self.walk_stream.wait_stream(self.gemm_stream)
             self.walker.set_positions(self.x0)
             check(L.nz_phase_init(_ptr(self.x0), self.N, _ptr(self.u), _ptr(self.gsum), self.W,
                                   _ptr(self.phase), _stream_ptr(self.device)), "nz_phase_init")
+            e = torch.cuda.Event()
+            e.record(self.walk_stream)
+            self.gemm_stream.wait_event(e)
+        self._idx = 0
 
     def block(self, t0, record_hook=None):
         """Walk and contract steps t0 .. t0 + K - 1 (fewer at the end)."""
         L = lib()
         k = min(self.K, self.T - t0)
-        s = _stream_ptr(self.device)
+        b = self._idx % self._nb
+        self._idx += 1
+        sa = ctypes.c_void_p(self.walk_stream.cuda_stream)
+        sb = ctypes.c_void_p(self.gemm_stream.cuda_stream)
         with torch.cuda.device(self.device):
+            if self._used[b]:  # block idx - 2's contraction still reads these buffers
+                self.walk_stream.wait_event(self.gemm_done[b])
             check(L.nz_walk_record(self.walker.handle, self.seed & (2**64 - 1), k, self.dt,
-                                   _ptr(self.rec), _ptr(self.slot_spin), s), "nz_walk_record")
+                                   _ptr(self.rec[b]), _ptr(self.slot_spin[b]), sa),
+                  "nz_walk_record")
             if record_hook is not None:
-                record_hook(t0, k, self.rec, self.slot_spin)
+                with torch.cuda.stream(self.walk_stream):
+                    record_hook(t0, k, self.rec[b], self.slot_spin[b])
+            self.walk_done[b].record(self.walk_stream)
+            self.gemm_stream.wait_event(self.walk_done[b])
+            if self.gemm_events is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e0.record(self.gemm_stream)
             check(L.nz_waveform_block(_ptr(self.g), self.T, _ptr(self.u), self.W, t0, self.K,
-                                      _ptr(self.bt_hi), _ptr(self.bt_lo), _ptr(self.bt_f32), s),
-                  "nz_waveform_block")
-            check(L.nz_phase_gemm(_ptr(self.rec), self.N, self.N, _ptr(self.slot_spin), self.K,
-                                  _ptr(self.bt_hi), _ptr(self.bt_lo), _ptr(self.bt_f32), self.W,
-                                  self.mode, _ptr(self.phase), s), "nz_phase_gemm")
+                                      _ptr(self.bt_hi[b]), _ptr(self.bt_lo[b]),
+                                      _ptr(self.bt_f32[b]), sb), "nz_waveform_block")
+            check(L.nz_phase_gemm(_ptr(self.rec[b]), self.N, self.N, _ptr(self.slot_spin[b]),
+                                  self.K, _ptr(self.bt_hi[b]), _ptr(self.bt_lo[b]),
+                                  _ptr(self.bt_f32[b]), self.W, self.mode, _ptr(self.phase), sb),
+                  "nz_phase_gemm")
+            if self.gemm_events is not None:
+                e1 = torch.cuda.Event(enable_timing=True)
+                e1.record(self.gemm_stream)
+                self.gemm_events.append((e0, e1))
+            self.gemm_done[b].record(self.gemm_stream)
+            self._used[b] = True
 
     def run(self, record_hook=None):
         self.reset()
         for t0 in range(0, self.T, self.K):
             self.block(t0, record_hook)
 
+    def finish(self):
+        """Make the current stream wait for every contraction issued so far."""
+        e = torch.cuda.Event()
+        e.record(self.gemm_stream)
+        torch.cuda.current_stream(self.device).wait_event(e)
+
     def signals(self):
         """Complex [W] = sum exp(i phase) / N (FP64 accumulation)."""
+        self.finish()
         with torch.cuda.device(self.device):
             check(lib().nz_phase_signals(_ptr(self.phase), self.N, self.W, _ptr(self.signals_out),
                                          _stream_ptr(self.device)), "nz_phase_signals")
@@ -122,4 +170,5 @@ class PrecisionRun:
         return (s[:, 0] + 1j * s[:, 1]) / self.N
 
     def close(self):
+        torch.cuda.synchronize(self.device)
         self.walker.close()

@@ -65,6 +65,7 @@ def test_precision_walk_against_oracle(oracle_lib):
     for mode in ("mfma", "fp32"):
         pr = PrecisionRun(sub, x0, seed, g, u, dt, K=K, mode=mode)
         pr.run(record_hook=hook if mode == "mfma" else None)
+        pr.finish()  # the contraction stream is done before the phases are read
         runs[mode] = (pr.phase.cpu().numpy().astype(np.float64), pr.signals())
         pr.close()
     # FP64 contraction of the recorded FP32 displacements: contraction error alone
@@ -93,3 +94,26 @@ def test_precision_walk_against_oracle(oracle_lib):
         print(f"{mode}: max |signal - FP64 oracle| = {es:.2e}")
         assert es < 1e-3
     assert np.abs(runs["mfma"][1] - runs["fp32"][1]).max() < 1e-4
+
+
+def test_overlapped_blocks_equal_serial():
+    """Contracting block b on a second stream while block b + 1 walks gives
+    the phases of the one-stream run, bit for bit (same kernels, same order
+    of the phase updates), also when a block ran before the reset."""
+    M, L = generate_beaded_axons(16, 30.0, seed=20261203)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    sub = geom.device_substrate(device=DEV)
+    N, T, dt, W, K, seed = 20000, 640, 0.02, 96, 64, 99
+    g, u, _ = precision_waveforms(W, T, dt, seed=3)
+    x0 = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=4)
+    out = {}
+    for overlap in (False, True):
+        pr = PrecisionRun(sub, x0, seed, g, u, dt, K=K, mode="mfma", overlap=overlap)
+        pr.reset()
+        pr.block(0)  # its contraction may still run when run() resets
+        pr.run()
+        pr.finish()
+        out[overlap] = (pr.phase.cpu().numpy().copy(), pr.signals())
+        pr.close()
+    assert np.array_equal(out[False][0], out[True][0])
+    assert np.array_equal(out[False][1], out[True][1])
