@@ -63,7 +63,7 @@ const char* nz_last_error_string(void);
  *                  lower wins, sim3d_kernel.cu:53-54)
  *   D_host, T2_host : nstructures + 1 entries, the box last (diffsim3d.py:95-103)
  *   L*       : box edges (um); periodic cube [-L/2, L/2]
- *   cell_size: target cell edge (um, 0 = 1.25); the grid tiles the box exactly
+ *   cell_size: target cell edge (um, 0 = 2.0); the grid tiles the box exactly
  *   margin   : per-cell list margin (um, 0 = 0.8); a sub-step that would
  *              leave the cell's box grown by the margin re-bins the spin
  * Synchronous: validates on the host, builds the lists, records and sort

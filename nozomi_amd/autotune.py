@@ -18,8 +18,8 @@ from . import runtime as rt
 _CACHE = {}
 
 
-def tune(geom, key=None, n=65536, steps=128, dt=0.01, cells=(1.25, 1.5, 2.0),
-         margins=(0.7, 0.8, 1.0), ks=(32, 64, 128), device=None, seed=1):
+def tune(geom, key=None, n=65536, steps=128, dt=0.01, cells=(1.5, 1.75, 2.0, 2.25),
+         margins=(0.8, 0.9), ks=(64, 96, 128), device=None, seed=1):
     """Returns {"cell": c, "margin": m, "k": k, "ns_per_spin_step": t,
     "table": [(t, c, m, k), ...]} for the geometry (a SimGeometry3D)."""
     if key is not None and key in _CACHE:

@@ -71,7 +71,7 @@ inline int grid_params(const double* spheres, int64_t S, int32_t nstruct, const 
       return -1;
     }
   }
-  if (cell_size <= 0) cell_size = 1.25;
+  if (cell_size <= 0) cell_size = 2.0;
   if (margin <= 0) margin = 0.8;
   std::memset(&g, 0, sizeof(g));
   g.nstruct = nstruct;

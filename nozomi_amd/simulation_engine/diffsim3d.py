@@ -86,7 +86,7 @@ class DiffSim3d:
     def _cell_size(self):
         if self.cell_size:
             return float(self.cell_size)
-        auto = 1.25
+        auto = 2.0
         L = min(self.geom.Lx, self.geom.Ly, self.geom.Lz)
         seg = L / max(self.nsegx, self.nsegy, self.nsegz)
         return max(min(auto, seg), 0.5) if max(self.nsegx, self.nsegy, self.nsegz) > 1 else auto
