@@ -276,6 +276,13 @@ def precision_form(args, geom, sub):
     return res
 
 
+def _progress(what):
+    """Progress lines on stderr for the long form measurements."""
+    def report(done, total):
+        print(f"[{what}] {done}/{total} done", file=sys.stderr, flush=True)
+    return report
+
+
 def forms(args):
     """Batched and served forms against back-to-back plain walker launches of
     the same simulations (each result checked against its run alone)."""
@@ -295,8 +302,8 @@ def batching_form(args):
     cache.prefetch(reqs)
     cap = sum(-(-r.n // 256) * 256 for r in reqs)
     run_serial(reqs[:4], cache=cache)  # warms the kernels
-    got, t_b = run_batched(reqs, capacity=cap, k=96, cache=cache)
-    alone, t_s = run_serial(reqs, cache=cache)
+    got, t_b = run_batched(reqs, capacity=cap, k=96, cache=cache, progress=_progress("batched"))
+    alone, t_s = run_serial(reqs, cache=cache, progress=_progress("serial"))
     diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
     ss = total_spin_steps(reqs)
     return {
@@ -317,10 +324,10 @@ def serving_form(args):
     pregenerate(reqs)
     cap = 16 * 1024 * 1024
     srv = Server(cap, hbm_budget=64 << 30, max_sims=256, k=64)
-    got, makespan = serve_trace(srv, reqs, arr)
+    got, makespan = serve_trace(srv, reqs, arr, progress=_progress("served"))
     built, reused = srv.subs.built, srv.subs.reused
     srv.close()
-    alone, t_s = run_serial(reqs)
+    alone, t_s = run_serial(reqs, progress=_progress("serial"))
     diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
     ss = total_spin_steps(reqs)
     lat_serial = fifo_latencies(arr, [alone[r.rid].latency for r in reqs])

@@ -224,13 +224,30 @@ class Server:
         self.batch.close()
 
 
-def serve_trace(server, requests, arrivals, poll=0.0):
+class _Progress:
+    """Calls fn(done, total) at most every `every` seconds (long runs report
+    as they go)."""
+
+    def __init__(self, fn, total, every=20.0):
+        self.fn, self.total, self.every = fn, total, every
+        self.last = time.perf_counter()
+
+    def __call__(self, done):
+        if self.fn is not None and time.perf_counter() - self.last >= self.every:
+            self.last = time.perf_counter()
+            self.fn(done, self.total)
+
+
+def serve_trace(server, requests, arrivals, poll=0.0, progress=None):
     """Drive `server` with requests arriving at the given times (seconds from
-    the start, sorted).  Returns (results by id, makespan seconds)."""
+    the start, sorted).  Returns (results by id, makespan seconds).
+    progress(done, total) is called every 20 s or so when given."""
     t0 = time.perf_counter()
     pending = list(zip(arrivals, range(len(requests))))
     heapq.heapify(pending)
+    report = _Progress(progress, len(requests))
     while pending or not server.idle():
+        report(len(server.results))
         now = time.perf_counter() - t0
         while pending and pending[0][0] <= now:
             at, i = heapq.heappop(pending)
@@ -246,7 +263,7 @@ def serve_trace(server, requests, arrivals, poll=0.0):
 
 
 def run_batched(requests, capacity, k=96, launches=2, device=None, graph=True, max_sims=256,
-                cache=None):
+                cache=None, progress=None):
     """Every request submitted at once, run to completion by one Server.
     Returns (results, seconds)."""
     srv = Server(capacity, hbm_budget=1 << 40, max_sims=max_sims, k=k, launches=launches,
@@ -256,24 +273,28 @@ def run_batched(requests, capacity, k=96, launches=2, device=None, graph=True, m
         t0 = time.perf_counter()
         for r in requests:
             srv.submit(r, now=t0)
+        report = _Progress(progress, len(requests))
         while not srv.idle():
             srv.step()
+            report(len(srv.results))
         torch.cuda.synchronize()
         return srv.results, time.perf_counter() - t0
     finally:
         srv.close()
 
 
-def run_serial(requests, device=None, cache=None):
+def run_serial(requests, device=None, cache=None, progress=None):
     """Back-to-back plain launches: one walker per request, one request at a
     time (substrates cached the same way).  Returns (results, seconds); a
     result's arrival/finished are the start and end of its own run."""
     device = rt._require_gpu(device)
     cache = cache or SubstrateCache(device, 1 << 40)
     results = {}
+    report = _Progress(progress, len(requests))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in requests:
+        report(len(results))
         t_start = time.perf_counter()
         sub, geom = cache.acquire(r.key, r.make_geometry)
         w = rt.Walker(sub, r.n)
