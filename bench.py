@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--precision-waveforms", type=int, default=1024)
     ap.add_argument("--only", choices=("precision", "batching", "serving"), default=None,
                     help="measure only this form and print its JSON object")
+    ap.add_argument("--workload", choices=("standard", "small"), default="standard",
+                    help="small: the CPU-runnable 16-axon configuration, GPU and FP64 oracle in full")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy a buffer of this many MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -85,6 +87,9 @@ def main():
         copy_probe(args.copy_probe)
         return
     import torch
+    if args.workload == "small":
+        print(json.dumps(small_workload(args)), flush=True)
+        return
     if args.only:
         if args.only == "precision":
             from nozomi_amd.substrate import generate_beaded_axons, load_geometry
@@ -274,6 +279,72 @@ def precision_form(args, geom, sub):
     res["max_signal_diff_mfma_vs_fp32"] = float(np.abs(sig["mfma"] - sig["fp32"]).max())
     res["max_signal_diff_overlap_vs_serial"] = float(np.abs(sig["mfma_overlap"] - sig["mfma"]).max())
     return res
+
+
+def small_workload(args):
+    """The CPU-runnable configuration in full on the GPU and with the FP64
+    oracle: 16 beaded axons in a 30 um periodic cube, 1e4 spins uniform in the
+    box, D = 2.0 um^2/ms, dt = 20 us, PGSE delta = 10 ms, Delta = 30 ms
+    (2000 steps), b = 1 ms/um^2, 6 directions."""
+    import torch
+    from nozomi_amd import runtime as rt
+    from nozomi_amd.protocol import PGSEShells, icosahedron_6
+    from nozomi_amd.substrate import generate_beaded_axons, load_geometry
+    from oracle.oracle import OracleSubstrate, build
+    dev = torch.device("cuda:0")
+    M, L = generate_beaded_axons(16, 30.0, seed=args.seed)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    prot = PGSEShells(10.0, 30.0, 0.02, [1.0], icosahedron_6())
+    N, T = 10000, prot.nsteps
+    sub = geom.device_substrate(device=dev)
+    start = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=args.seed + 1)
+    w = rt.Walker(sub, N, sort_interval=args.sort)
+    g = torch.from_numpy(prot.g.copy()).to(dev)
+    ga = torch.from_numpy(prot.garea.copy()).to(dev)
+    dirs = torch.from_numpy(prot.dirs.astype(np.float32)).to(dev)
+    scales = torch.from_numpy(prot.scales.astype(np.float32)).to(dev)
+    mom = torch.empty((3, N), dtype=torch.float32, device=dev)
+    times = []
+    for rep in range(4):  # the first pass warms up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        w.set_positions(start)
+        w.walk(args.seed, T, prot.dt, g=g, garea=ga, t2=False)
+        w.read(mom=mom)
+        sums = rt.signal_sums(mom, dirs, scales).cpu().numpy()
+        times.append(time.perf_counter() - t0)
+    sg = (sums[:, 0] + 1j * sums[:, 1]) / N
+    pos_g, disp_g = w.read_f64()
+    w.close()
+    build()
+    orc = OracleSubstrate(geom.spheres, geom.nstructures, geom.structure_D(), geom.structure_T2(),
+                          (L, L, L))
+    p = start.cpu().numpy().astype(np.float64)
+    pos, pos0, sigw, mo = p.copy(), p.copy(), np.ones(N), np.zeros((3, N))
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    t0 = time.perf_counter()
+    orc.walk(pos, pos0, sigw, seed=args.seed, step0=0, nsteps=T, dt=prot.dt,
+             g=prot.g.astype(np.float64), garea=prot.garea.astype(np.float64), mom=mo,
+             nthreads=threads)
+    t_cpu = time.perf_counter() - t0
+    so = np.exp(1j * prot.scales[None, :] * (mo.T @ prot.dirs.astype(np.float64))).mean(0)
+    d = pos_g - pos
+    d -= L * np.round(d / L)
+    err = np.maximum(np.abs(d).max(0), np.abs(disp_g - (pos - pos0)).max(0))
+    t_gpu = float(np.median(times[1:]))
+    return {
+        "workload": "16 axons, 30 um box, 1e4 spins, 2000 steps of 20 us, PGSE 10/30 ms, "
+                    "b = 1 ms/um^2, 6 directions",
+        "gpu_seconds": t_gpu, "gpu_spin_steps_per_s": N * T / t_gpu,
+        "cpu_seconds": t_cpu, "cpu_spin_steps_per_s": N * T / t_cpu, "cpu_threads": threads,
+        "cpu_kind": "port (FP64 oracle, OpenMP)",
+        "signals_gpu": [float(x) for x in sg.real], "signals_oracle": [float(x) for x in so.real],
+        "signal_max_abs_diff": float(np.abs(sg - so).max()),
+        "signal_sampling_se": float(np.sqrt(np.maximum(1 - np.abs(so) ** 2, 0) / N).mean()),
+        "free_diffusion_exp_minus_bD": float(np.exp(-prot.b_per_signal[0] * 2.0)),
+        "trajectory_fraction_within_1e-5_um": float(np.mean(err <= 1e-5)),
+        "trajectory_error_median_um": float(np.median(err)),
+    }
 
 
 def _progress(what):
