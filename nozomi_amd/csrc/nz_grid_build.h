@@ -41,6 +41,7 @@ struct HostGrid {
     g.cell_start = start.data();
     g.cell_rank = rank.data();
     g.cand = cand.data();
+    g.cand_bytes = (int32_t)(sizeof(float4) * cand.size());
     g.sph = sph.data();
     g.sph_sid = sid.data();
     g.D = D.data();
@@ -121,8 +122,8 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
     std::vector<int64_t> fill;
     if (pass == 1) {
       for (int64_t c = 0; c < ncells; ++c) cnt[c + 1] += cnt[c];
-      if (cnt[ncells] > (int64_t)INT32_MAX) {
-        err = "candidate lists exceed 2^31 entries; lower cell_size or margin";
+      if (cnt[ncells] > kMaxCandEntries) {
+        err = "candidate lists exceed 2^26 entries (2 GiB of records); lower cell_size or margin";
         return -1;
       }
       for (int64_t c = 0; c < ncells; ++c)
@@ -131,8 +132,8 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
                 " spheres; lower cell_size or margin";
           return -1;
         }
-      // padding records after the last list: the scan loads up to
-      // kCandPad records past the end of a list
+      // padding records after the last list: the scan loads ahead, up to
+      // kCandPad - 1 records past the end of a list
       out.cand.assign(2 * ((size_t)cnt[ncells] + kCandPad), make_float4(0, 0, 0, 0));
       fill.assign(cnt.begin(), cnt.end() - 1);
     }

@@ -258,9 +258,9 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
   NZ_HIP(hipGetLastError());
   unsigned long long hs[4];
   NZ_HIP(hipMemcpy(hs, stats, sizeof(hs), hipMemcpyDeviceToHost));
-  if (hs[1] > (unsigned long long)INT32_MAX - 8)
-    return fail(-1, "nz_substrate_create: candidate lists exceed 2^31 entries; lower cell_size or "
-                    "margin");
+  if (hs[1] > (unsigned long long)kMaxCandEntries)
+    return fail(-1, "nz_substrate_create: candidate lists exceed 2^26 entries (2 GiB of records); "
+                    "lower cell_size or margin");
   if (hs[0] > (unsigned long long)kMaxListLen)
     return fail(-1, "nz_substrate_create: a cell lists more than " + std::to_string(kMaxListLen) +
                         " spheres; lower cell_size or margin");
@@ -311,6 +311,7 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
   g.cell_start = start;
   g.cell_rank = rank;
   g.cand = cand;
+  g.cand_bytes = (int32_t)(sizeof(float4) * 2 * ((size_t)total + kCandPad));
   g.sph = sph;
   g.sph_sid = sid;
   g.D = Dd;

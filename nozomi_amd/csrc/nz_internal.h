@@ -17,8 +17,12 @@ constexpr float kLarge = 1e6f;
 constexpr int kMaxIter = 100;
 // Longest candidate list of a cell (the FP32 scan keys a list position in 8 bits).
 constexpr int kMaxListLen = 255;
-// Zero records after the last candidate list (the scans load ahead).
-constexpr int kCandPad = 8;
+// Zero records after the last candidate list (the scans load up to 15
+// records past a list's end).
+constexpr int kCandPad = 16;
+// Most candidate entries of a substrate: the record array (32 bytes an entry,
+// padding included) stays below 2 GiB, the range of the scan's buffer loads.
+constexpr int64_t kMaxCandEntries = ((int64_t)1 << 26) - kCandPad;
 
 // Device view of a substrate.  The box [-L/2, L/2] per axis is tiled by
 // ncell[a] cells of edge h[a]; one halo cell on each side (local index 0 and
@@ -48,6 +52,7 @@ struct Grid {
   // so a wavefront's spins tend to scan lists of similar length
   const int32_t* cell_rank;
   const float4* cand;         // [2 * (entries + kCandPad)]: padding records at the end
+  int32_t cand_bytes;         // size of cand in bytes (< 2^31)
   // every sphere, absolute, FP64: the exact re-solves and the rare sub-step
   // that leaves the grid's lists
   const double4* sph;         // (cx, cy, cz, r)

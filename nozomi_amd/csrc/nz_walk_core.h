@@ -289,6 +289,45 @@ NZ_HD inline int cand_sphere(const Grid& g, int e) {
   return (int)f2u(g.cand[2 * (int64_t)e + 1].w);
 }
 
+// Loads of the records of one candidate list (position e = record cs + e).
+// On the device these are buffer loads on the whole record array: a plain
+// load carried from one loop trip to the next is merged by the compiler into
+// a load at its use, which would undo the loads issued a trip ahead; buffer
+// loads are kept where they are written.  Every offset stays inside the
+// array (padding records included).
+#if defined(__HIP_DEVICE_COMPILE__)
+// Keeps the instruction scheduler from moving code across this point (the
+// loads issued above stay ahead of the arithmetic below).
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+typedef unsigned int nz_u32x4 __attribute__((ext_vector_type(4)));
+struct RecLoader {
+  __amdgpu_buffer_rsrc_t r;
+  int base;  // byte offset of the list's first record
+  __device__ RecLoader(const Grid& g, int cs)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void*)g.cand, (short)0, g.cand_bytes, 0x00020000)),
+        base(cs * 32) {}
+  __device__ float4 ld(int voff, int soff) const {
+    const nz_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                       __uint_as_float(v.w));
+  }
+  // position e, the same on every lane (its offset goes in a scalar register)
+  __device__ float4 a_u(int e) const { return ld(base, e * 32); }
+  // position e of this lane
+  __device__ float4 a(int e) const { return ld(base + e * 32, 0); }
+  __device__ float4 b(int e) const { return ld(base + e * 32 + 16, 0); }
+};
+#else
+inline void sched_fence() {}
+struct RecLoader {
+  const float4* p;
+  RecLoader(const Grid& g, int cs) : p(g.cand + 2 * (int64_t)cs) {}
+  float4 a_u(int e) const { return p[2 * e]; }
+  float4 a(int e) const { return p[2 * e]; }
+  float4 b(int e) const { return p[2 * e + 1]; }
+};
+#endif
+
 // Every candidate of the list through the full visit, with the next record's
 // loads issued before the current one is evaluated.  Loads past the list
 // stay in bounds: the record array ends with padding records.
@@ -319,25 +358,24 @@ NZ_HD inline bool reachable(const ScanIn& in, const float4 A) {
   return dd <= fmaf(A.w, in.reach1, in.reach0);
 }
 
-// Lowest set position of the two masks, cleared; `none` when both are empty.
-NZ_HD inline int pop_flag(uint32_t& m0, uint32_t& m1, int none) {
-  if (m0 != 0u) {
-    const int q = __builtin_ctz(m0);
-    m0 &= m0 - 1u;
-    return q;
-  }
-  if (m1 != 0u) {
-    const int q = 32 + __builtin_ctz(m1);
-    m1 &= m1 - 1u;
-    return q;
-  }
-  return none;
+// Reach bits (bit i: candidate A_i) of four records.
+NZ_HD inline uint32_t reach4(const ScanIn& in, const float4 A0, const float4 A1, const float4 A2,
+                             const float4 A3) {
+  return (uint32_t)reachable(in, A0) | ((uint32_t)reachable(in, A1) << 1) |
+         ((uint32_t)reachable(in, A2) << 2) | ((uint32_t)reachable(in, A3) << 3);
+}
+
+// Lowest set position of the mask, cleared; `none` when it is empty.
+NZ_HD inline int pop_flag(uint64_t& m, int none) {
+  if (m == 0u) return none;
+  const int q = __builtin_ctzll(m);
+  m &= m - 1u;
+  return q;
 }
 
 // The candidate scan over the list [cs, ce).  A reach pass over every
-// candidate (its first float4 only; four independent tests per trip, the
-// next four records in flight) marks the spheres the segment can touch as
-// bits of two 32-bit masks (the loop position is the same on every lane);
+// candidate (its first float4 only) marks the spheres the segment can touch
+// as bits of a 64-bit mask (the loop position is the same on every lane);
 // only those get the full visit, a few per list against ~15 candidates.
 // Against visiting every candidate the exit key and the structure id are
 // equal; the entry key can only differ by entries beyond jr (the skipped
@@ -346,36 +384,41 @@ NZ_HD inline int pop_flag(uint32_t& m0, uint32_t& m1, int none) {
 // Lists longer than 64 take the full visit for every candidate.
 NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, ScanKeys& k) {
   const int n = ce - cs;
-  const float4* rec = g.cand + 2 * (int64_t)cs;
   if (n > 64) {
-    scan_all(rec, n, in, k);
+    scan_all(g.cand + 2 * (int64_t)cs, n, in, k);
     return;
   }
-  uint32_t m0 = 0u, m1 = 0u;
-  float4 A0 = rec[0], A1 = rec[2], A2 = rec[4], A3 = rec[6];
-  for (int e = 0; e < n; e += 4) {  // four candidates per trip, loaded one trip ahead
-    const float4 N0 = rec[8], N1 = rec[10], N2 = rec[12], N3 = rec[14];
-    const uint32_t b = (reachable(in, A0) ? 1u : 0u) | ((e + 1 < n && reachable(in, A1)) ? 2u : 0u) |
-                       ((e + 2 < n && reachable(in, A2)) ? 4u : 0u) |
-                       ((e + 3 < n && reachable(in, A3)) ? 8u : 0u);
-    if (e < 32) m0 |= b << e;
-    else m1 |= b << (e - 32);
-    A0 = N0;
-    A1 = N1;
-    A2 = N2;
-    A3 = N3;
-    rec += 8;
-  }
+  const RecLoader rl(g, cs);
+  // eight candidates per trip in two groups of four, each group's records
+  // loaded while the other group is tested (the two register sets alternate
+  // roles, so nothing is copied).  Positions past the list (loads stay below
+  // n + 16) are tested too and masked off after the loop.
+  uint64_t m = 0u;
+  float4 A0 = rl.a_u(0), A1 = rl.a_u(1), A2 = rl.a_u(2), A3 = rl.a_u(3);
+  int e = 0;
+  do {
+    const float4 N0 = rl.a_u(e + 4), N1 = rl.a_u(e + 5), N2 = rl.a_u(e + 6), N3 = rl.a_u(e + 7);
+    sched_fence();
+    uint32_t b = reach4(in, A0, A1, A2, A3);
+    A0 = rl.a_u(e + 8);
+    A1 = rl.a_u(e + 9);
+    A2 = rl.a_u(e + 10);
+    A3 = rl.a_u(e + 11);
+    sched_fence();
+    b |= reach4(in, N0, N1, N2, N3) << 4;
+    m |= (uint64_t)b << e;
+    e += 8;
+  } while (e < n);
+  m &= n >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << n) - 1u);
   // the marked records, each loaded while the previous one is visited; the
   // position n (none left) loads the record after the list, not visited
-  rec = g.cand + 2 * (int64_t)cs;
-  int q = pop_flag(m0, m1, n);
-  float4 A = rec[2 * q];
-  float4 B = rec[2 * q + 1];
+  int q = pop_flag(m, n);
+  float4 A = rl.a(q);
+  float4 B = rl.b(q);
   while (q < n) {
-    const int qn = pop_flag(m0, m1, n);
-    const float4 An = rec[2 * qn];
-    const float4 Bn = rec[2 * qn + 1];
+    const int qn = pop_flag(m, n);
+    const float4 An = rl.a(qn);
+    const float4 Bn = rl.b(qn);
     visit(in, A, B, (uint32_t)(q + 1), k);
     q = qn;
     A = An;
