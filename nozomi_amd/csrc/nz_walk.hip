@@ -185,9 +185,15 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
         started = true;
       }
       if (q.pending >= 0) step_draw(q, P.seed, spin, step);
+      // the step's gradient samples, loaded before the sub-step so that their
+      // latency hides behind it (used only if the sub-step ends the step)
+      float G = 0.0f, Ga = 0.0f;
+      if (P.gph) {
+        G = P.gph[t - P.base];
+        Ga = P.garea[t - P.base];
+      }
+      sched_fence();
       if (substep(g, s, q, P.seed, spin, step, P.dt, do_t2)) {
-        const float G = P.gph ? P.gph[t - P.base] : 0.0f;
-        const float Ga = P.gph ? P.garea[t - P.base] : 0.0f;
         step_end(g, s, G, Ga);
         if (kTrack && P.rec) {  // displacement since the start, for the blocked phase contraction
           const int64_t r = (t - P.mom_base) * 3;

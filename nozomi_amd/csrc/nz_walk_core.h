@@ -318,7 +318,7 @@ struct RecLoader {
   __device__ float4 b(int e) const { return ld(base + e * 32 + 16, 0); }
 };
 #else
-inline void sched_fence() {}
+NZ_HD inline void sched_fence() {}
 struct RecLoader {
   const float4* p;
   RecLoader(const Grid& g, int cs) : p(g.cand + 2 * (int64_t)cs) {}
