@@ -317,6 +317,8 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
   g.D = Dd;
   g.T2 = T2d;
   g.inv_T2 = invd;
+  if (int rc = dev_alloc(s, 1, &s->g_dev)) return rc;
+  NZ_HIP(hipMemcpy(s->g_dev, &g, sizeof(Grid), hipMemcpyHostToDevice));
   nz_substrate_info_t& inf = s->info;
   inf.nspheres = S;
   inf.nstructures = nstruct;

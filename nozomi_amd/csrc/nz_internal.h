@@ -65,6 +65,7 @@ struct Grid {
 
 struct Substrate {
   Grid g;
+  Grid* g_dev = nullptr;  // a copy of g in device memory
   nz_substrate_info_t info;
   std::vector<void*> allocations;
   int device = 0;

@@ -54,7 +54,9 @@ struct Walker {
 };
 
 struct WalkArgs {
-  Grid g;
+  // the grid in device memory: read through a pointer its fields are loaded
+  // where used instead of all occupying scalar registers from kernel entry
+  const Grid* g;
   SlotState st;
   int64_t n;
   uint64_t seed;
@@ -117,12 +119,13 @@ constexpr int kMomChunk = 256;
 #endif
 
 // Per-launch parameters of one simulation's walk.
+// Step numbers are 32-bit on the device (the host keeps them below 2^32).
 struct StepParams {
   uint64_t seed;
-  int64_t base;      // global step of gph[0] / garea[0]
-  int64_t target;    // the wavefront runs until all its spins reached this step
-  int64_t limit;     // a spin may run ahead up to this step
-  int64_t mom_base;  // global step of stepmom[0] for this launch
+  uint32_t base;      // global step of gph[0] / garea[0]
+  uint32_t target;    // the wavefront runs until all its spins reached this step
+  uint32_t limit;     // a spin may run ahead up to this step
+  uint32_t mom_base;  // global step of stepmom[0] for this launch
   double dt;
   const float* gph;
   const float* garea;
@@ -155,7 +158,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   s.nguard = 0;
   const bool do_t2 = (P.flags & NZ_WALK_T2) != 0;
   const uint32_t spin = active ? (uint32_t)st.spin_id[ii] : 0u;
-  int64_t t = active ? (int64_t)st.tstep[ii] : P.target;
+  uint32_t t = active ? st.tstep[ii] : P.target;
   int u0[3] = {0, 0, 0};
   double p0[3] = {0.0, 0.0, 0.0};
   if (kTrack && P.stepmom) {
@@ -179,7 +182,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   while (__any(t < P.target || started)) {
     const bool behind = __any(t < P.target);
     if (active && (started || (t < P.limit && (t < P.target || behind)))) {
-      const uint32_t step = (uint32_t)t;
+      const uint32_t step = t;
       if (!started) {
         step_begin(g, s, q, P.dt);
         started = true;
@@ -196,7 +199,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
       if (substep(g, s, q, P.seed, spin, step, P.dt, do_t2)) {
         step_end(g, s, G, Ga);
         if (kTrack && P.rec) {  // displacement since the start, for the blocked phase contraction
-          const int64_t r = (t - P.mom_base) * 3;
+          const int64_t r = (int64_t)(t - P.mom_base) * 3;
 #pragma unroll
           for (int a = 0; a < 3; ++a) P.rec[(r + a) * ld + i] = (float)lane_disp(g, s, u0, p0, a);
         }
@@ -217,7 +220,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   if (kTrack && P.stepmom) {  // block partials -> FP64 sums
     __syncthreads();
     for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x)
-      atomicAdd(&P.stepmom[(P.mom_base - P.base) * 6 + k], (double)smom[k]);
+      atomicAdd(&P.stepmom[(int64_t)(P.mom_base - P.base) * 6 + k], (double)smom[k]);
   }
 
   if (!active) return;
@@ -229,7 +232,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   }
   st.logsig[i] = s.logsig;
   st.sid[i] = (int32_t)s.sid;
-  st.tstep[i] = (uint32_t)t;
+  st.tstep[i] = t;
   if (st.nsub) st.nsub[i] += s.nsub;
   if (st.nguard) st.nguard[i] += s.nguard;
 }
@@ -239,9 +242,9 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A
   __shared__ float smom[kTrack ? kMomChunk * 6 : 1];
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
   const int64_t i = blk * blockDim.x + threadIdx.x;
-  const StepParams P{A.seed,  A.base,    A.target, A.limit, A.mom_base, A.dt,
-                     A.gph,   A.garea,   A.stepmom, A.flags, A.rec};
-  walk_slot<kTrack>(A.g, A.st, A.n, i, i < A.n, P, smom);
+  const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
+                     (uint32_t)A.mom_base, A.dt, A.gph, A.garea, A.stepmom, A.flags, A.rec};
+  walk_slot<kTrack>(*A.g, A.st, A.n, i, i < A.n, P, smom);
 }
 
 // Absolute start positions (spin order) -> resident state (slot = spin).
@@ -757,8 +760,8 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR batch_walk_kernel(const SimD
   if (!d.active) return;
   const int64_t i = pb * kBlk + threadIdx.x;
   const int64_t target = min(d.step + d.k, d.nsteps);
-  const StepParams P{d.seed, 0, target, d.nsteps, 0, d.dt, d.gph, d.garea, nullptr, d.flags,
-                     nullptr};
+  const StepParams P{d.seed, 0u, (uint32_t)target, (uint32_t)d.nsteps, 0u, d.dt, d.gph, d.garea,
+                     nullptr, d.flags, nullptr};
   walk_slot<false>(d.g, st, ld, i, st.spin_id[i] >= 0, P, nullptr);
 }
 
@@ -1070,7 +1073,7 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
     if (w->sort_interval > 0) chunk = std::min<int64_t>(chunk, w->sort_interval - w->since_sort);
     if (stepmom) chunk = std::min<int64_t>(chunk, kMomChunk);
     WalkArgs a;
-    a.g = w->sub->g;
+    a.g = w->sub->g_dev;
     a.st = w->buf[w->cur];
     a.n = w->n;
     a.seed = seed;
@@ -1115,7 +1118,7 @@ int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, flo
   if (w->sort_interval > 0 && w->since_sort >= w->sort_interval)
     if (int rc = sort_spins(w, s)) return rc;
   WalkArgs a;
-  a.g = w->sub->g;
+  a.g = w->sub->g_dev;
   a.st = w->buf[w->cur];
   a.n = w->n;
   a.seed = seed;
