@@ -54,7 +54,6 @@ struct Walker {
 };
 
 struct WalkArgs {
-  Grid g;  // by value: the tracking walk's kernel reads it from its arguments
   SlotState st;
   int64_t n;
   uint64_t seed;
@@ -236,18 +235,20 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
 }
 
 template <bool kTrack>
-// The plain walk reads the grid through `gp`, a device copy: its fields are
-// then loaded where used instead of all occupying scalar registers from
-// kernel entry (fewer spills).  The tracking walk stores to global memory in
-// its loop, after which loads through a pointer would be repeated, so it
-// reads the grid from its arguments.
-__global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A, const Grid* gp) {
+// The grid comes as a device copy (`gp` and `gpr` both point to it).  The
+// plain walk reads it through `gp`: its fields are then loaded where used
+// instead of all occupying scalar registers from kernel entry (fewer
+// spills).  The tracking walk stores to global memory in its loop, after which
+// loads through a plain pointer would be repeated; it reads through the
+// __restrict__ `gpr`, whose loads the compiler may keep.
+__global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A, const Grid* gp,
+                                                              const Grid* __restrict__ gpr) {
   __shared__ float smom[kTrack ? kMomChunk * 6 : 1];
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
   const int64_t i = blk * blockDim.x + threadIdx.x;
   const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
                      (uint32_t)A.mom_base, A.dt, A.gph, A.garea, A.stepmom, A.flags, A.rec};
-  walk_slot<kTrack>(kTrack ? A.g : *gp, A.st, A.n, i, i < A.n, P, smom);
+  walk_slot<kTrack>(kTrack ? *gpr : *gp, A.st, A.n, i, i < A.n, P, smom);
 }
 
 // Absolute start positions (spin order) -> resident state (slot = spin).
@@ -1076,7 +1077,6 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
     if (w->sort_interval > 0) chunk = std::min<int64_t>(chunk, w->sort_interval - w->since_sort);
     if (stepmom) chunk = std::min<int64_t>(chunk, kMomChunk);
     WalkArgs a;
-    a.g = w->sub->g;
     a.st = w->buf[w->cur];
     a.n = w->n;
     a.seed = seed;
@@ -1098,10 +1098,10 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
     }
     if (stepmom)
       hipLaunchKernelGGL(walk_kernel<true>, dim3(blocks_for(w->n)), dim3(256), 0, s, a,
-                         (const Grid*)w->sub->g_dev);
+                         (const Grid*)w->sub->g_dev, (const Grid*)w->sub->g_dev);
     else
       hipLaunchKernelGGL(walk_kernel<false>, dim3(blocks_for(w->n)), dim3(256), 0, s, a,
-                         (const Grid*)w->sub->g_dev);
+                         (const Grid*)w->sub->g_dev, (const Grid*)w->sub->g_dev);
     NZ_HIP(hipGetLastError());
     if (w->timing) NZ_HIP(hipEventRecord(e1, s));
     target += chunk;
@@ -1123,7 +1123,6 @@ int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, flo
   if (w->sort_interval > 0 && w->since_sort >= w->sort_interval)
     if (int rc = sort_spins(w, s)) return rc;
   WalkArgs a;
-  a.g = w->sub->g;
   a.st = w->buf[w->cur];
   a.n = w->n;
   a.seed = seed;
@@ -1139,7 +1138,7 @@ int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, flo
   a.xcd_chunk = w->xcd_chunk;
   a.rec = rec;
   hipLaunchKernelGGL(walk_kernel<true>, dim3(blocks_for(w->n)), dim3(256), 0, s, a,
-                     (const Grid*)w->sub->g_dev);
+                     (const Grid*)w->sub->g_dev, (const Grid*)w->sub->g_dev);
   NZ_HIP(hipGetLastError());
   NZ_HIP(hipMemcpyAsync(slot_spin, w->buf[w->cur].spin_id, sizeof(int32_t) * w->n,
                         hipMemcpyDeviceToDevice, s));
