@@ -78,15 +78,22 @@ constexpr int kMaxK3 = 384;  // K <= 128: A operands held in registers
 //   current chunk's MFMAs run, and one barrier per chunk suffices.
 // - D goes through LDS so each phase row is read-modified-written by 64 lanes
 //   at once (256 contiguous bytes); each wave touches only its own 16 rows.
-// 37 KB of LDS and 168 VGPRs: three workgroups per CU.
+// 33 KB of LDS and 168 VGPRs: three workgroups per CU.
 // MFMA operand layout (lane l of 64): A row l % 16, k 8 (l / 16) .. + 8;
 // B column l % 16, the same k; D column l % 16, rows 4 (l / 16) .. + 4.
+// Position of k (a multiple of 8) in LDS row `col` of a waveform chunk: rows
+// of 32 BF16 (64 bytes, four 16-byte pieces) with the piece index XORed with
+// bits 1-2 of the row, so that the MFMA B reads (16 rows, one piece each) and
+// the chunk stores (4 pieces of 16 rows) fall in distinct LDS banks (the
+// bank-conflict counter reads ~0 against 44% of LDS cycles with padded rows).
+__device__ __forceinline__ int sbk(int col, int k) { return 8 * ((k >> 3) ^ ((col >> 1) & 3)); }
+
 template <int NKC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin,
                        const uint16_t* bt_hi, const uint16_t* bt_lo, int32_t W, float* phase) {
   constexpr int K3 = NKC * kChunk;
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][kTileW][kChunk + 8];  // [buf][hi, lo]
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][kTileW][kChunk];  // [buf][hi, lo]
   __shared__ float sC[kTileS][kTileW + 1];
   const int64_t s0 = (int64_t)blockIdx.x * kTileS;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -127,8 +134,8 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
 #pragma unroll
     for (int kc = 0; kc < NKC; ++kc) {
       // buffer `buf` was last read two chunks ago, before the previous barrier
-      __builtin_memcpy(&sB[buf][0][ww][q], &nh, 16);
-      __builtin_memcpy(&sB[buf][1][ww][q], &nl, 16);
+      __builtin_memcpy(&sB[buf][0][ww][sbk(ww, q)], &nh, 16);
+      __builtin_memcpy(&sB[buf][1][ww][sbk(ww, q)], &nl, 16);
       {
         const int nw0 = (kc + 1 < NKC) ? w0 : w0 + kTileW;
         const int nkc = (kc + 1 < NKC) ? kc + 1 : 0;
@@ -147,8 +154,8 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
       for (int c = 0; c < 4; ++c) {
         const int col = c * 16 + (lane & 15);
         bf16x8 bh, bl;
-        __builtin_memcpy(&bh, &sB[buf][0][col][kl], 16);
-        __builtin_memcpy(&bl, &sB[buf][1][col][kl], 16);
+        __builtin_memcpy(&bh, &sB[buf][0][col][sbk(col, kl)], 16);
+        __builtin_memcpy(&bl, &sB[buf][1][col][sbk(col, kl)], 16);
         acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bh, acc[c], 0, 0, 0);
         acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bl, acc[c], 0, 0, 0);
         acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kc], bh, acc[c], 0, 0, 0);
