@@ -98,7 +98,8 @@ def test_oracle_is_chaotic(c0):
     assert np.mean(err[res[0][1] <= 10] < 1e-7) > 0.99   # collision-poor spins do not
 
 
-def test_two_phase_scan_equals_full_scan(c0):
+@pytest.mark.parametrize("cell", [1.25, 4.0])
+def test_two_phase_scan_equals_full_scan(c0, cell):
     """The reach pass may only skip spheres that cannot change the sub-step:
     against visiting every candidate of the list, the exit key and the
     structure id are equal, the entry key is equal whenever either is a root
@@ -106,9 +107,11 @@ def test_two_phase_scan_equals_full_scan(c0):
     guards differ only by skipped out-of-reach ones.  Uniform probes and
     probes on and near sphere surfaces (inside and outside the tolerance
     shells), segments from 1e-4 um to the list margin, every remaining
-    fraction, every assumed structure id."""
+    fraction, every assumed structure id.  4 um cells give lists of up to 66
+    entries: every tail length of the eight-candidate reach trips, and lists
+    past the 64-entry two-phase limit."""
     g, _ = c0
-    cw = CpuWalker(g, cell=1.25, margin=0.8)
+    cw = CpuWalker(g, cell=cell, margin=0.8)
     rng = np.random.default_rng(7)
     L = g.Lx
     sp = g.spheres

@@ -57,14 +57,18 @@ def test_seed_matches_oracle(c0_geometry, oracle_lib):
         assert np.mean(first_t < first_n) > 0.9995
 
 
-def test_trajectory_parity_1000_steps(c0_geometry, oracle_lib):
+@pytest.mark.parametrize("cell", [None, 4.0])
+def test_trajectory_parity_1000_steps(c0_geometry, oracle_lib, cell):
     """Per-spin FP64 state against the oracle every 100 steps.  Wall
     collisions amplify any rounding difference exponentially (see
     tests/test_cpu_core.py::test_oracle_is_chaotic), so the 1e-5 um bar is
-    applied to spins with at most 60 scatter sub-steps and the rest reported."""
+    applied to spins with at most 60 scatter sub-steps and the rest reported.
+    Cells: the default edge, and 4 um (candidate lists of up to 66 entries,
+    past the reach pass's 64-entry limit)."""
     g = c0_geometry
     N, dt, chunk, nchunks = 4096, 0.02, 100, 10
-    sim = DiffSim3d(g, N, counters=True)
+    print(f"cell {cell}", flush=True)
+    sim = DiffSim3d(g, N, counters=True, cell_size=cell)
     sim.setup(initstates_int=1234)
     p_gpu0 = sim.spins_d.get().astype(np.float64)
     orc = _oracle_for(g, oracle_lib)
