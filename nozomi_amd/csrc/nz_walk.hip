@@ -103,9 +103,9 @@ constexpr int kMomChunk = 256;
 // meanwhile spins that are ahead keep stepping up to `limit` (the end of the
 // walk), so collision-poor spins do not idle at every launch boundary.
 // Register budget of the walk kernel: waves per SIMD the compiler must fit
-// (0 = unconstrained).  Three waves (168 VGPRs, a few spills) beat the
-// unconstrained two (172 VGPRs) by 1.23x on the 200-axon substrate; four
-// waves (128 VGPRs, more spills) measure the same as three.
+// (0 = unconstrained).  Three waves (164 VGPRs, no spills) beat the
+// unconstrained two by 1.23x on the 200-axon substrate; four waves (128
+// VGPRs, 72 bytes per lane of spills) run 9% slower than three.
 #ifndef NZ_WALK_WAVES_PER_EU
 #define NZ_WALK_WAVES_PER_EU 3
 #endif
