@@ -100,6 +100,34 @@ def test_trajectory_parity_1000_steps(c0_geometry, oracle_lib, cell):
     assert np.abs(gs[few] / sig[few] - 1).max() < 1e-5
 
 
+@pytest.mark.parametrize("dt,nsteps", [(0.3, 40), (1.0, 20)])
+def test_large_steps_fp64_fallbacks(c0_geometry, oracle_lib, dt, nsteps):
+    """Jumps of 1.1-2 um rms (beyond the 0.8 um list margin): most sub-steps
+    leave their cell's safety box, so the walk re-bins and runs the FP64
+    27-cell and all-sphere scans; trajectories still follow the oracle."""
+    g = c0_geometry
+    N = 2048
+    print(f"dt {dt} ms, {nsteps} steps", flush=True)
+    sim = DiffSim3d(g, N, counters=True)
+    sim.setup(initstates_int=1234)
+    p0 = sim.spins_d.get().astype(np.float64)
+    orc = _oracle_for(g, oracle_lib)
+    pos, pos0, sig = p0.copy(), p0.copy(), np.ones(N)
+    nsub_o = np.zeros(N, dtype=np.int64)
+    sim.run(nsteps, dt)
+    orc.walk(pos, pos0, sig, seed=1234, step0=0, nsteps=nsteps, dt=dt, nsub=nsub_o)
+    gp, gd = sim.walker.read_f64()
+    _, nguard = sim.walker.counters()
+    err = np.maximum(np.abs(_min_image(gp - pos, g.Lx)).max(axis=0),
+                     np.abs(gd - (pos - pos0)).max(axis=0))
+    few = nsub_o <= 60
+    print(f"  within 1e-5: all {np.mean(err < 1e-5):.4f}, <=60 collisions "
+          f"{np.mean(err[few] < 1e-5):.4f}; FP64 scans {int(nguard.sum())}", flush=True)
+    assert nguard.sum() > 10000
+    assert np.mean(err[few] < 1e-5) > 0.995
+    assert np.mean(err < 1e-5) > 0.98
+
+
 def test_phase_signal_parity(c0_geometry, oracle_lib):
     g = c0_geometry
     prot = PGSEShells(10.0, 30.0, 0.02, [1.0], icosahedron_6())
