@@ -12,6 +12,7 @@ import torch
 
 from nozomi_amd.protocol import PGSEShells, icosahedron_6
 from nozomi_amd.simulation_engine.diffsim3d import DiffSim3d, DwiSim3d
+from nozomi_amd.simulation_engine.geometry import SimGeometry3D, Structure3D
 from nozomi_amd.substrate import generate_beaded_axons, load_geometry
 
 pytestmark = pytest.mark.gpu
@@ -126,6 +127,39 @@ def test_large_steps_fp64_fallbacks(c0_geometry, oracle_lib, dt, nsteps):
     assert nguard.sum() > 10000
     assert np.mean(err[few] < 1e-5) > 0.995
     assert np.mean(err < 1e-5) > 0.98
+
+
+def test_non_cubic_box_parity(oracle_lib):
+    """A 24 x 30 x 36 um box (a different cell count and wrap per axis)
+    holding 40 structures of three overlapping spheres each (structures may
+    overlap one another): 300 steps against the oracle."""
+    rng = np.random.default_rng(31)
+    Lx, Ly, Lz = 24.0, 30.0, 36.0
+    g = SimGeometry3D(Lx, Ly, Lz, 2.0, 100.0, 1.0)
+    for _ in range(40):
+        c = (rng.random(3) - 0.5) * (np.array([Lx, Ly, Lz]) - 6.0)
+        off = rng.normal(size=(3, 3)) * 0.6
+        r = rng.uniform(0.8, 1.6, 3)
+        g.add_structure(Structure3D(c[0] + off[0], c[1] + off[1], c[2] + off[2], r, 2.0, 80.0, 1.0))
+    N, dt, T = 2048, 0.02, 300
+    sim = DiffSim3d(g, N, counters=True)
+    sim.setup(initstates_int=77)
+    p0 = sim.spins_d.get().astype(np.float64)
+    orc = _oracle_for(g, oracle_lib)
+    pos, pos0, sig = p0.copy(), p0.copy(), np.ones(N)
+    nsub_o = np.zeros(N, dtype=np.int64)
+    sim.run(T, dt)
+    orc.walk(pos, pos0, sig, seed=77, step0=0, nsteps=T, dt=dt, nsub=nsub_o)
+    gp, gd = sim.walker.read_f64()
+    d = gp - pos
+    d -= np.array([[Lx], [Ly], [Lz]]) * np.round(d / np.array([[Lx], [Ly], [Lz]]))
+    err = np.maximum(np.abs(d).max(axis=0), np.abs(gd - (pos - pos0)).max(axis=0))
+    few = nsub_o <= 60
+    print(f"non-cubic: within 1e-5 all {np.mean(err < 1e-5):.4f}, <=60 collisions "
+          f"{np.mean(err[few] < 1e-5):.4f} ({few.sum()} spins); collisions {int(nsub_o.sum())}", flush=True)
+    assert nsub_o.sum() > 1000
+    assert np.mean(err[few] < 1e-5) > 0.995
+    assert np.mean(err < 1e-5) > 0.95
 
 
 def test_phase_signal_parity(c0_geometry, oracle_lib):
