@@ -162,6 +162,37 @@ def test_non_cubic_box_parity(oracle_lib):
     assert np.mean(err < 1e-5) > 0.95
 
 
+def test_compartment_d_t2_parity(oracle_lib):
+    """Intra-axonal D = 1.0, T2 = 80 ms against extra-axonal D = 2.5, T2 =
+    120 ms: the jump scale and the T2 rate follow the structure id of every
+    sub-step (cached between changes); 500 steps, with T2, against the
+    oracle."""
+    M, L = generate_beaded_axons(16, 30.0, seed=20261203)
+    g = load_geometry(M, L, D=1.0, T2=80.0, D_box=2.5, T2_box=120.0)
+    N, dt, T = 2048, 0.02, 500
+    sim = DiffSim3d(g, N, counters=True)
+    sim.setup(initstates_int=99)
+    p0 = sim.spins_d.get().astype(np.float64)
+    orc = _oracle_for(g, oracle_lib)
+    pos, pos0, sig = p0.copy(), p0.copy(), np.ones(N)
+    nsub_o = np.zeros(N, dtype=np.int64)
+    sim.run(T, dt)
+    orc.walk(pos, pos0, sig, seed=99, step0=0, nsteps=T, dt=dt, nsub=nsub_o)
+    gp, gd = sim.walker.read_f64()
+    err = np.maximum(np.abs(_min_image(gp - pos, L)).max(axis=0),
+                     np.abs(gd - (pos - pos0)).max(axis=0))
+    few = nsub_o <= 60
+    gs = sim.sig_d.get().ravel().astype(np.float64)
+    print(f"compartments: within 1e-5 all {np.mean(err < 1e-5):.4f}, <=60 collisions "
+          f"{np.mean(err[few] < 1e-5):.4f}; T2 weight rel err {np.abs(gs[few] / sig[few] - 1).max():.2e}",
+          flush=True)
+    assert np.mean(err[few] < 1e-5) > 0.995
+    assert np.mean(err < 1e-5) > 0.95
+    assert np.abs(gs[few] / sig[few] - 1).max() < 1e-5
+    # both compartments' rates are in use
+    assert sig.min() < np.exp(-T * dt / 80.0) * 1.001 and sig.max() > np.exp(-T * dt / 120.0) * 0.999
+
+
 def test_phase_signal_parity(c0_geometry, oracle_lib):
     g = c0_geometry
     prot = PGSEShells(10.0, 30.0, 0.02, [1.0], icosahedron_6())
