@@ -59,11 +59,14 @@ class Batch:
         float32 device tensors [nsteps] (kept alive until remove); pos:
         optional [3, n] start positions (device tensor), else seeding into
         `targets` (structure ids; None = every structure and the box)."""
+        if (g is None) != (garea is None):
+            raise ValueError("g and garea go together")
         if g is not None:
             if g.dtype != torch.float32 or garea.dtype != torch.float32:
                 raise ValueError("g and garea must be float32 tensors")
             if g.numel() < nsteps or garea.numel() < nsteps:
                 raise ValueError("g and garea need nsteps entries")
+        pg, pa = _ptr(g, self.device, "g"), _ptr(garea, self.device, "garea")
         t = None
         if pos is None:
             if targets is None:
@@ -76,8 +79,9 @@ class Batch:
         sim = ctypes.c_int32(-1)
         with torch.cuda.device(self.device):
             rc = lib().nz_batch_add(self.handle, sub.handle, int(n), int(seed) & (2**64 - 1),
-                                    int(nsteps), float(dt), _ptr(g), _ptr(garea),
-                                    _lib.NZ_WALK_T2 if t2 else 0, int(k), _ptr(pos),
+                                    int(nsteps), float(dt), pg, pa,
+                                    _lib.NZ_WALK_T2 if t2 else 0, int(k),
+                                    _ptr(pos, self.device, "pos"),
                                     None if t is None else t.ctypes.data_as(ctypes.c_void_p),
                                     0 if t is None else t.size, ctypes.byref(sim),
                                     _stream_ptr(self.device))
@@ -125,17 +129,32 @@ class Batch:
 
     def read(self, sim, n, pos=None, pos0=None, sig=None, mom=None):
         """Fill the given device tensors in spin order ([3, n] / [n])."""
+        def out(t, name, numel):
+            if t is not None and (t.dtype != torch.float32 or t.numel() < numel):
+                raise ValueError(f"{name} must be a float32 tensor with at least {numel} entries")
+            return _ptr(t, self.device, name)
+        args = (out(pos, "pos", 3 * n), out(pos0, "pos0", 3 * n), out(sig, "sig", n),
+                out(mom, "mom", 3 * n))
         with torch.cuda.device(self.device):
-            check(lib().nz_batch_read(self.handle, int(sim), _ptr(pos), _ptr(pos0), _ptr(sig),
-                                      _ptr(mom), _stream_ptr(self.device)), "nz_batch_read")
+            check(lib().nz_batch_read(self.handle, int(sim), *args, _stream_ptr(self.device)),
+                  "nz_batch_read")
 
     def signals(self, sim, dirs, scales=None, out=None):
         """FP64 [W, 2] (sum cos, sum sin) of scale_w u_w . m over the
         simulation's spins."""
+        if dirs.dim() != 2 or dirs.shape[0] != 3 or dirs.dtype != torch.float32:
+            raise ValueError("dirs must be a float32 tensor of shape (3, W)")
         W = dirs.shape[1]
+        if scales is not None and (scales.dtype != torch.float32 or scales.numel() != W):
+            raise ValueError("scales must be a float32 tensor of W entries")
         out = out if out is not None else torch.empty((W, 2), dtype=torch.float64,
                                                       device=self.device)
+        if out.dtype != torch.float64 or out.numel() < 2 * W:
+            raise ValueError("out must be a float64 tensor of 2*W entries")
+        dirs = dirs.contiguous()
+        args = (_ptr(dirs, self.device, "dirs"), _ptr(scales, self.device, "scales"),
+                _ptr(out, self.device, "out"))
         with torch.cuda.device(self.device):
-            check(lib().nz_batch_signals(self.handle, int(sim), int(W), _ptr(dirs), _ptr(scales),
-                                         _ptr(out), _stream_ptr(self.device)), "nz_batch_signals")
+            check(lib().nz_batch_signals(self.handle, int(sim), int(W), *args,
+                                         _stream_ptr(self.device)), "nz_batch_signals")
         return out

@@ -21,8 +21,11 @@ constexpr int kMaxListLen = 255;
 // records past a list's end).
 constexpr int kCandPad = 16;
 // Most candidate entries of a substrate: the record array (32 bytes an entry,
-// padding included) stays below 2 GiB, the range of the scan's buffer loads.
-constexpr int64_t kMaxCandEntries = ((int64_t)1 << 26) - kCandPad;
+// padding included) stays below 2 GiB, the range of the scan's buffer loads
+// and of the int32 byte size it is described by.
+constexpr int64_t kMaxCandEntries = ((int64_t)1 << 26) - kCandPad - 1;
+static_assert((kMaxCandEntries + kCandPad) * 32 < ((int64_t)1 << 31),
+              "record array size must fit in int32");
 
 // Device view of a substrate.  The box [-L/2, L/2] per axis is tiled by
 // ncell[a] cells of edge h[a]; one halo cell on each side (local index 0 and

@@ -220,14 +220,19 @@ __global__ void __launch_bounds__(256) phase_gemm_fp32_kernel(
 }
 
 // phase[i, w] = (x0_i . u_w) * sum_t g_w(t): the start position's share.
+// Grid: x over spins in groups of kInitSpins, y over waveforms in tiles of
+// 256 (one thread per waveform, so each spin's row is written coalesced);
+// no dimension grows with n * W.
+constexpr int kInitSpins = 16;
 __global__ void __launch_bounds__(256) phase_init_kernel(const float* x0, int64_t n, const float* u,
                                                          const float* gsum, int32_t W,
                                                          float* phase) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * W) return;
-  const int64_t i = e / W;
-  const int w = (int)(e % W);
-  phase[e] = (x0[i] * u[w] + x0[n + i] * u[W + w] + x0[2 * n + i] * u[2 * W + w]) * gsum[w];
+  const int w = blockIdx.y * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const float ux = u[w], uy = u[W + w], uz = u[2 * W + w], gs = gsum[w];
+  const int64_t i0 = (int64_t)blockIdx.x * kInitSpins;
+  for (int64_t i = i0; i < i0 + kInitSpins && i < n; ++i)
+    phase[i * W + w] = (x0[i] * ux + x0[n + i] * uy + x0[2 * n + i] * uz) * gs;
 }
 
 // out[2w], out[2w+1] += sum over a block of spins of cos / sin phase[i, w]
@@ -312,8 +317,10 @@ int nz_phase_gemm(const float* rec, int64_t ld, int64_t nslots, const int32_t* s
 int nz_phase_init(const float* x0, int64_t n, const float* u, const float* gsum, int32_t W,
                   float* phase, void* stream) {
   if (!x0 || !u || !gsum || !phase || n <= 0 || W <= 0) return fail(-1, "nz_phase_init: bad arguments");
-  const int64_t total = n * W;
-  hipLaunchKernelGGL(phase_init_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+  const int64_t gx = (n + kInitSpins - 1) / kInitSpins, gy = ((int64_t)W + 255) / 256;
+  if (gx > (int64_t)INT32_MAX || gy > 65535)
+    return fail(-1, "nz_phase_init: n or W too large for one launch");
+  hipLaunchKernelGGL(phase_init_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0,
                      (hipStream_t)stream, x0, n, u, gsum, W, phase);
   NZ_HIP(hipGetLastError());
   return 0;

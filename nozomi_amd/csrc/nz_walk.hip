@@ -16,10 +16,10 @@ namespace nz {
 // the read-back.
 struct SlotState {
   int32_t* spin_id;  // [n]
-  int32_t* ucell;    // [3,n] unwrapped cell index
-  double* rel;       // [3,n] position relative to the cell's lower corner
-  int32_t* ucell0;   // [3,n] start cell
-  double* rel0;      // [3,n] start position relative to that cell
+  int32_t* cell;     // [3,n] local index of the cell holding x
+  double* x;         // [3,n] absolute position (wrapped as the reference wraps)
+  int32_t* wrap;     // [3,n] net +L shifts of the start position
+  double* x0;        // [3,n] start position
   int32_t* sid;      // [n]
   float* logsig;     // [n]
   float* mom;        // [3,n]
@@ -87,7 +87,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb, int64_t chun
   return g0 + (b % kXcd) * chunk + (b - g0) / kXcd;
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
+__device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -139,16 +139,16 @@ struct StepParams {
 template <bool kTrack>
 __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, int64_t ld,
                                           int64_t i, bool active, const StepParams P,
-                                          float* smom) {
+                                          double* smom) {
   const int64_t ii = active ? i : 0;
   Lane s;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    s.u[a] = active ? st.ucell[a * ld + ii] : 0;
-    s.p[a] = active ? st.rel[a * ld + ii] : 0.0;
+    s.x[a] = active ? st.x[a * ld + ii] : 0.0;
+    s.l[a] = active ? st.cell[a * ld + ii] : 1;
+    s.w[a] = active ? st.wrap[a * ld + ii] : 0;
     s.m[a] = (active && P.gph) ? st.mom[a * ld + ii] : 0.0f;
   }
-  lane_load_cells(g, s);
   s.logsig = active ? st.logsig[ii] : 0.0f;
   s.sid = active ? (float)st.sid[ii] : (float)g.nstruct;
   s.nsub = 0;
@@ -156,18 +156,14 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   const bool do_t2 = (P.flags & NZ_WALK_T2) != 0;
   const uint32_t spin = active ? (uint32_t)st.spin_id[ii] : 0u;
   uint32_t t = active ? st.tstep[ii] : P.target;
-  int u0[3] = {0, 0, 0};
-  double p0[3] = {0.0, 0.0, 0.0};
+  double x0[3] = {0.0, 0.0, 0.0};
   if (kTrack && P.stepmom) {
-    for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x) smom[k] = 0.0f;
+    for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x) smom[k] = 0.0;
     __syncthreads();
   }
   if (kTrack && (P.stepmom || P.rec)) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      u0[a] = active ? st.ucell0[a * ld + ii] : 0;
-      p0[a] = active ? st.rel0[a * ld + ii] : 0.0;
-    }
+    for (int a = 0; a < 3; ++a) x0[a] = active ? st.x0[a * ld + ii] : 0.0;
   }
 
   bool started = false;
@@ -198,15 +194,16 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
         if (kTrack && P.rec) {  // displacement since the start, for the blocked phase contraction
           const int64_t r = (int64_t)(t - P.mom_base) * 3;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) P.rec[(r + a) * ld + i] = (float)lane_disp(g, s, u0, p0, a);
+          for (int a = 0; a < 3; ++a) P.rec[(r + a) * ld + i] = (float)lane_disp(g, s, x0, a);
         }
-        if (kTrack && P.stepmom) {  // moment sums (sim3d_kernel.cu:191-237)
+        if (kTrack && P.stepmom) {  // moment sums (sim3d_kernel.cu:191-237), FP64 throughout
           const int k = (int)(t - P.mom_base) * 6;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            const float d = (float)lane_disp(g, s, u0, p0, a);
-            atomicAdd(&smom[k + a], d * d);
-            atomicAdd(&smom[k + 3 + a], d * d * d * d);
+            const double d = lane_disp(g, s, x0, a);
+            const double d2 = d * d;
+            atomicAdd(&smom[k + a], d2);
+            atomicAdd(&smom[k + 3 + a], d2 * d2);
           }
         }
         ++t;
@@ -217,14 +214,15 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   if (kTrack && P.stepmom) {  // block partials -> FP64 sums
     __syncthreads();
     for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x)
-      atomicAdd(&P.stepmom[(int64_t)(P.mom_base - P.base) * 6 + k], (double)smom[k]);
+      atomicAdd(&P.stepmom[(int64_t)(P.mom_base - P.base) * 6 + k], smom[k]);
   }
 
   if (!active) return;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    st.ucell[a * ld + i] = s.u[a];
-    st.rel[a * ld + i] = s.p[a];
+    st.x[a * ld + i] = s.x[a];
+    st.cell[a * ld + i] = s.l[a];
+    st.wrap[a * ld + i] = s.w[a];
     if (P.gph) st.mom[a * ld + i] = s.m[a];
   }
   st.logsig[i] = s.logsig;
@@ -243,7 +241,8 @@ template <bool kTrack>
 // __restrict__ `gpr`, whose loads the compiler may keep.
 __global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A, const Grid* gp,
                                                               const Grid* __restrict__ gpr) {
-  __shared__ float smom[kTrack ? kMomChunk * 6 : 1];
+  // per-step moment partials of the block (12 KiB at kMomChunk = 256)
+  __shared__ double smom[kTrack ? kMomChunk * 6 : 1];
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
   const int64_t i = blk * blockDim.x + threadIdx.x;
   const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
@@ -262,10 +261,10 @@ __global__ void __launch_bounds__(256) init_kernel(const Grid g, const SlotState
   st.spin_id[i] = (int32_t)i;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    st.ucell[a * n + i] = s.u[a];
-    st.ucell0[a * n + i] = s.u[a];
-    st.rel[a * n + i] = s.p[a];
-    st.rel0[a * n + i] = s.p[a];
+    st.cell[a * n + i] = s.l[a];
+    st.wrap[a * n + i] = 0;
+    st.x[a * n + i] = s.x[a];
+    st.x0[a * n + i] = s.x[a];
     st.mom[a * n + i] = 0.0f;
   }
   st.sid[i] = (int32_t)s.sid;
@@ -286,13 +285,16 @@ __global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
-__device__ __forceinline__ int32_t wrapped_cell(const Grid& g, const int32_t* ucell, int64_t n,
-                                                int64_t i) {
+// In-box cell of a slot (its local cell index, the halo folded onto the
+// nearest in-box cell: a spin exactly on the x = +L/2 face, or the rare one a
+// step carried beyond the box).
+__device__ __forceinline__ int32_t box_cell(const Grid& g, const int32_t* cell, int64_t n,
+                                            int64_t i) {
   int c[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    int w = ucell[a * n + i] % g.ncell[a];
-    c[a] = w < 0 ? w + g.ncell[a] : w;
+    const int l = cell[a * n + i] - 1;
+    c[a] = l < 0 ? 0 : (l >= g.ncell[a] ? g.ncell[a] - 1 : l);
   }
   return (c[2] * g.ncell[1] + c[1]) * g.ncell[0] + c[0];
 }
@@ -331,7 +333,7 @@ __global__ void __launch_bounds__(256) sort_keys_kernel(const Grid g, const Slot
   // the cells' sort ranks (regions, then list length): spins with similar
   // collision rates and candidate counts share wavefronts
   const int64_t ncells = (int64_t)g.ncell[0] * g.ncell[1] * g.ncell[2];
-  const int32_t k = valid ? g.cell_rank[wrapped_cell(g, st.ucell, n, i)] +
+  const int32_t k = valid ? g.cell_rank[box_cell(g, st.cell, n, i)] +
                                 (st.sid[i] >= g.nstruct ? (int32_t)ncells : 0)
                           : -1;
   if (valid) keys[i] = k;
@@ -477,10 +479,10 @@ __global__ void __launch_bounds__(256) sort_scatter_kernel(const SlotState src, 
   dst.spin_id[j] = src.spin_id[i];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    dst.ucell[a * n + j] = src.ucell[a * n + i];
-    dst.rel[a * n + j] = src.rel[a * n + i];
-    dst.ucell0[a * n + j] = src.ucell0[a * n + i];
-    dst.rel0[a * n + j] = src.rel0[a * n + i];
+    dst.cell[a * n + j] = src.cell[a * n + i];
+    dst.x[a * n + j] = src.x[a * n + i];
+    dst.wrap[a * n + j] = src.wrap[a * n + i];
+    dst.x0[a * n + j] = src.x0[a * n + i];
     dst.mom[a * n + j] = src.mom[a * n + i];
   }
   dst.sid[j] = src.sid[i];
@@ -502,31 +504,29 @@ __global__ void __launch_bounds__(256) read_kernel(const Grid g, const SlotState
   const int64_t k = st.spin_id[i];
   if (k < 0 || k >= n) return;  // padding slots hold -1
   Lane s;
-  int u0[3];
-  double p0[3];
+  double x0[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    s.u[a] = st.ucell[a * ld + i];
-    s.p[a] = st.rel[a * ld + i];
-    u0[a] = st.ucell0[a * ld + i];
-    p0[a] = st.rel0[a * ld + i];
+    s.x[a] = st.x[a * ld + i];
+    s.l[a] = st.cell[a * ld + i];
+    s.w[a] = st.wrap[a * ld + i];
+    x0[a] = st.x0[a * ld + i];
   }
-  lane_load_cells(g, s);
   if (pos || pos0) {
-    float x[3], x0[3];
-    lane_outputs(g, s, u0, p0, x, x0);
+    float xf[3], x0f[3];
+    lane_outputs(g, s, x0, xf, x0f);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (pos) pos[a * n + k] = x[a];
-      if (pos0) pos0[a * n + k] = x0[a];
+      if (pos) pos[a * n + k] = xf[a];
+      if (pos0) pos0[a * n + k] = x0f[a];
     }
   }
   if (sig) sig[k] = expf(st.logsig[i]);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (mom) mom[a * n + k] = st.mom[a * ld + i];
-    if (pos_d) pos_d[a * n + k] = cell_lo(g, a, s.l[a]) + s.p[a];
-    if (disp_d) disp_d[a * n + k] = lane_disp(g, s, u0, p0, a);
+    if (pos_d) pos_d[a * n + k] = s.x[a];
+    if (disp_d) disp_d[a * n + k] = lane_disp(g, s, x0, a);
   }
   if (nsub) nsub[k] = st.nsub ? st.nsub[i] : 0;
   if (nguard) nguard[k] = st.nguard ? st.nguard[i] : 0;
@@ -535,20 +535,27 @@ __global__ void __launch_bounds__(256) read_kernel(const Grid g, const SlotState
 __global__ void __launch_bounds__(256) disp_moments_kernel(const Grid g, const SlotState st,
                                                            int64_t n, double* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  float v[6] = {0, 0, 0, 0, 0, 0};
+  double v[6] = {0, 0, 0, 0, 0, 0};
   if (i < n) {
+    Lane s;
+    double x0[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const float d = (float)((double)(st.ucell[a * n + i] - st.ucell0[a * n + i]) * g.hd[a] +
-                              (st.rel[a * n + i] - st.rel0[a * n + i]));
+      s.x[a] = st.x[a * n + i];
+      s.w[a] = st.wrap[a * n + i];
+      x0[a] = st.x0[a * n + i];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double d = lane_disp(g, s, x0, a);
       v[a] = d * d;
       v[3 + a] = v[a] * v[a];
     }
   }
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    const float s = wave_sum(v[q]);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&out[q], (double)s);
+    const double s = wave_sum_d(v[q]);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&out[q], s);
   }
 }
 
@@ -709,10 +716,10 @@ __global__ void __launch_bounds__(256) batch_init_kernel(const Grid g, const Slo
   st.spin_id[i] = (int32_t)e;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    st.ucell[a * ld + i] = s.u[a];
-    st.ucell0[a * ld + i] = s.u[a];
-    st.rel[a * ld + i] = s.p[a];
-    st.rel0[a * ld + i] = s.p[a];
+    st.cell[a * ld + i] = s.l[a];
+    st.wrap[a * ld + i] = 0;
+    st.x[a * ld + i] = s.x[a];
+    st.x0[a * ld + i] = s.x[a];
     st.mom[a * ld + i] = 0.0f;
   }
   st.sid[i] = (int32_t)s.sid;
@@ -738,13 +745,7 @@ __global__ void __launch_bounds__(256) batch_keys_kernel(const SimDesc* descs,
     if (st.spin_id[i] < 0) {
       key = d.key0 + 2 * d.ncells;
     } else {
-      int c[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const int w = st.ucell[a * ld + i] % g.ncell[a];
-        c[a] = w < 0 ? w + g.ncell[a] : w;
-      }
-      const int64_t cell = ((int64_t)c[2] * g.ncell[1] + c[1]) * g.ncell[0] + c[0];
+      const int64_t cell = box_cell(g, st.cell, ld, i);
       key = d.key0 + g.cell_rank[cell] + (st.sid[i] >= g.nstruct ? d.ncells : 0);
     }
   }
@@ -917,10 +918,10 @@ static SlotState batch_view(const Batch* b, int32_t sim) {
   SlotState v = b->buf[b->cur];
   const int64_t o = b->jobs[sim].b0 * kBlk;
   v.spin_id += o;
-  v.ucell += o;
-  v.rel += o;
-  v.ucell0 += o;
-  v.rel0 += o;
+  v.cell += o;
+  v.x += o;
+  v.wrap += o;
+  v.x0 += o;
   v.sid += o;
   v.logsig += o;
   v.mom += o;
@@ -954,13 +955,13 @@ int nz_walker_create(nz_substrate_t sub, int64_t n, uint32_t flags, nz_walker_t*
     if ((rc = walker_alloc(w, sizeof(int32_t) * n, &p))) break;
     s.spin_id = (int32_t*)p;
     if ((rc = walker_alloc(w, sizeof(int32_t) * 3 * n, &p))) break;
-    s.ucell = (int32_t*)p;
+    s.cell = (int32_t*)p;
     if ((rc = walker_alloc(w, sizeof(double) * 3 * n, &p))) break;
-    s.rel = (double*)p;
+    s.x = (double*)p;
     if ((rc = walker_alloc(w, sizeof(int32_t) * 3 * n, &p))) break;
-    s.ucell0 = (int32_t*)p;
+    s.wrap = (int32_t*)p;
     if ((rc = walker_alloc(w, sizeof(double) * 3 * n, &p))) break;
-    s.rel0 = (double*)p;
+    s.x0 = (double*)p;
     if ((rc = walker_alloc(w, sizeof(int32_t) * n, &p))) break;
     s.sid = (int32_t*)p;
     if ((rc = walker_alloc(w, sizeof(float) * n, &p))) break;
@@ -1243,13 +1244,13 @@ int nz_batch_create(int64_t capacity, int32_t max_sims, nz_batch_t* out) {
     if ((rc = A(sizeof(int32_t) * n))) break;
     s.spin_id = (int32_t*)p;
     if ((rc = A(sizeof(int32_t) * 3 * n))) break;
-    s.ucell = (int32_t*)p;
+    s.cell = (int32_t*)p;
     if ((rc = A(sizeof(double) * 3 * n))) break;
-    s.rel = (double*)p;
+    s.x = (double*)p;
     if ((rc = A(sizeof(int32_t) * 3 * n))) break;
-    s.ucell0 = (int32_t*)p;
+    s.wrap = (int32_t*)p;
     if ((rc = A(sizeof(double) * 3 * n))) break;
-    s.rel0 = (double*)p;
+    s.x0 = (double*)p;
     if ((rc = A(sizeof(int32_t) * n))) break;
     s.sid = (int32_t*)p;
     if ((rc = A(sizeof(float) * n))) break;

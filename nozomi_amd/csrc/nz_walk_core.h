@@ -11,10 +11,13 @@
 //    band of its decision threshold is re-solved in FP64, and the two winning
 //    roots are always re-solved in FP64 with the reference's formulas, so the
 //    branch taken and the contact point are those of a double-precision walk;
-//  * the spin position (relative to its cell), the jump vector and the jump
-//    bookkeeping are FP64: per sub-step, not per candidate, so they cost
-//    little, and they keep round-off far below the chaotic growth of
-//    perturbations at repeated wall collisions;
+//  * the spin position, the jump vector and the jump bookkeeping are FP64 in
+//    the reference's absolute frame, with its operation order and without
+//    FP contraction: per sub-step, not per candidate, so they cost little.
+//    Wall collisions amplify any rounding difference exponentially, so the
+//    FP64 steps reproduce the double-precision restatement of the reference
+//    operation for operation; only the FP32 candidate scan (which merely
+//    picks the deciding spheres) works in cell-relative coordinates;
 //  * the structure pass and the collision pass are fused into one scan that
 //    assumes the structure id of the previous sub-step and re-runs only when
 //    the id changed, which is exact with respect to the two-pass order;
@@ -47,9 +50,16 @@ NZ_HD inline double MAXd(double a, double b) { return (a < b) ? b : a; }
 constexpr float kGuardInside = 1e-5f;
 constexpr float kGuardDisc = 2e-5f;
 
+// Round-off band of an FP32 root as the scan keys it (the root itself to
+// ~1e-5 relative, the key's 15 mantissa bits to 3e-5; a root near zero to
+// ~1e-6 absolute): two roots closer than this cannot be ordered in FP32.
+constexpr float kTieRel = 2e-4f;
+constexpr float kTieAbs = 1e-5f;
+
 // findStepSizeToSphereInsideTol (sim3d_kernel.cu:77-116) in FP64.
 NZ_HD inline double step_inside_d(double dx, double dy, double dz, double r, double jx, double jy,
                                   double jz) {
+#pragma clang fp contract(off)
   const double tol = (double)kTol;
   if ((dx * dx + dy * dy + dz * dz - r * r) > 0) return (double)kLarge;
   const double a = jx * jx + jy * jy + jz * jz;
@@ -66,6 +76,7 @@ NZ_HD inline double step_inside_d(double dx, double dy, double dz, double r, dou
 // findStepSizeToSphereOutsideTol (sim3d_kernel.cu:118-160) in FP64.
 NZ_HD inline double step_outside_d(double dx, double dy, double dz, double r, double jx, double jy,
                                    double jz) {
+#pragma clang fp contract(off)
   const double tol = (double)kTol;
   const double a = jx * jx + jy * jy + jz * jz;
   const double b = 2 * (jx * dx + jy * dy + jz * dz);
@@ -87,6 +98,7 @@ NZ_HD inline double step_outside_d(double dx, double dy, double dz, double r, do
 // arithmetic as step_inside_d / step_outside_d.
 NZ_HD inline double root_d(bool inside, double dx, double dy, double dz, double r, double jx,
                            double jy, double jz) {
+#pragma clang fp contract(off)
   const double tol = (double)kTol;
   const double d2r2 = dx * dx + dy * dy + dz * dz - r * r;
   if (inside && d2r2 > 0) return (double)kLarge;
@@ -113,6 +125,11 @@ struct ScanAcc {
   float fout;     // closest entry fraction among lower-id spheres
   int kin;        // sphere index of fin (-1: none above 0)
   int kout;       // sphere index of fout (-1: none)
+  // runners-up: the second furthest exit and the second closest entry (an
+  // FP32 root cannot order two roots closer than its own round-off, so both
+  // are re-solved in FP64 when they are that close)
+  float fin2, fout2;
+  int kin2, kout2;  // -1: none
   bool any_in;    // inside some relevant sphere
   bool exact;     // a candidate fell in a guard band: redo the scan in FP64
 };
@@ -138,6 +155,8 @@ struct ScanIn {
 struct ScanKeys {
   uint32_t kin;    // 0: no relevant containing sphere
   uint32_t kout;   // 0xffffffff: no obstacle
+  uint32_t kin2;   // second largest exit key (0: none)
+  uint32_t kout2;  // second smallest entry key (0xffffffff: none)
   float sid_new;   // smallest id of a containing sphere (box id if none)
   int32_t gcnt;    // candidates in a guard band
   uint32_t gpos;   // position + 1 of the last of them
@@ -155,35 +174,61 @@ NZ_HD inline float u2f(uint32_t u) {
   return f;
 }
 
+// Unsigned max / min (one v_max_u32 / v_min_u32 each).
+NZ_HD inline uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+NZ_HD inline uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
 NZ_HD inline void acc_reset(const Grid& g, ScanAcc& acc) {
   acc.sid_new = (float)g.nstruct;
   acc.fin = 0.0f;
   acc.fout = kLarge;
   acc.kin = -1;
   acc.kout = -1;
+  acc.fin2 = 0.0f;
+  acc.fout2 = kLarge;
+  acc.kin2 = -1;
+  acc.kout2 = -1;
   acc.any_in = false;
   acc.exact = false;
 }
 
-// One candidate in FP64 (sphere index k; p, j relative to the cell corner
-// lo): the exact structure and collision tests of the reference.
-NZ_HD inline void visit_exact(const Grid& g, const double p[3], const double j[3],
-                              const double lo[3], float sid, int k, ScanAcc& acc) {
+// One candidate in FP64 (sphere index k; x the absolute position, j the
+// jump): the exact structure and collision tests of the reference
+// (isInsideSphere and the two root routines, sim3d_kernel.cu:72-160,380-455).
+NZ_HD inline void visit_exact(const Grid& g, const double x[3], const double j[3], float sid, int k,
+                              ScanAcc& acc) {
+#pragma clang fp contract(off)
   const double4 s = g.sph[k];
   const float sidc = (float)g.sph_sid[k];
-  const double dx = p[0] - (s.x - lo[0]);
-  const double dy = p[1] - (s.y - lo[1]);
-  const double dz = p[2] - (s.z - lo[2]);
+  const double dx = x[0] - s.x;
+  const double dy = x[1] - s.y;
+  const double dz = x[2] - s.z;
   const bool inside = (dx * dx + dy * dy + dz * dz - s.w * s.w) < 0;
   if (inside && sidc < acc.sid_new) acc.sid_new = sidc;
   if (sidc > sid) return;
   if (inside) {
     const float f = (float)step_inside_d(dx, dy, dz, s.w, j[0], j[1], j[2]);
     acc.any_in = true;
-    if (acc.fin < f) { acc.fin = f; acc.kin = k; }
+    if (acc.fin < f) {
+      acc.fin2 = acc.fin;
+      acc.kin2 = acc.kin;
+      acc.fin = f;
+      acc.kin = k;
+    } else if (acc.fin2 < f) {
+      acc.fin2 = f;
+      acc.kin2 = k;
+    }
   } else if (sidc < sid) {
     const float f = (float)step_outside_d(dx, dy, dz, s.w, j[0], j[1], j[2]);
-    if (f < acc.fout) { acc.fout = f; acc.kout = k; }
+    if (f < acc.fout) {
+      acc.fout2 = acc.fout;
+      acc.kout2 = acc.kout;
+      acc.fout = f;
+      acc.kout = k;
+    } else if (f < acc.fout2) {
+      acc.fout2 = f;
+      acc.kout2 = k;
+    }
   }
 }
 
@@ -231,8 +276,13 @@ NZ_HD inline void visit(const ScanIn& in, const float4 A, const float4 B, uint32
   const uint32_t key_out = (f2u(f_out) & kKeyMask) | pos1;
   const bool use_in = relevant & inside & !guard;                     // :428-440
   const bool use_out = !inside & (B.z < in.sid) & !guard;             // :441-454
-  k.kin = (use_in & (key_in > k.kin)) ? key_in : k.kin;
-  k.kout = (use_out & (key_out < k.kout)) ? key_out : k.kout;
+  // best and runner-up by integer max / min (0 and 0xffffffff never win)
+  const uint32_t ki = use_in ? key_in : 0u;
+  const uint32_t ko = use_out ? key_out : 0xffffffffu;
+  k.kin2 = umax(k.kin2, umin(k.kin, ki));
+  k.kin = umax(k.kin, ki);
+  k.kout2 = umin(k.kout2, umax(k.kout, ko));
+  k.kout = umin(k.kout, ko);
 }
 
 NZ_HD inline int cell_id(const Grid& g, int lx, int ly, int lz) {
@@ -255,21 +305,21 @@ NZ_HD inline bool in_safety(const Grid& g, const double p[3], const double j[3],
   return ok;
 }
 
-// Move the lane's frame to the cell holding p (in-step re-bin).  Returns
-// false when that cell is outside the grid's halo.
-NZ_HD inline bool rebin(const Grid& g, double p[3], int l[3], int u[3]) {
+// Move the lane to the cell holding its position (in-step re-bin); rel, the
+// position relative to the cell corner, follows.  Returns false when that
+// cell is outside the grid's halo.
+NZ_HD inline bool rebin(const Grid& g, double rel[3], int l[3]) {
   bool ok = true;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int k = (int)floor(p[a] / g.hd[a]);
+    const int k = (int)floor(rel[a] / g.hd[a]);
     if (k != 0) {
       const int nl = l[a] + k;
       if (nl < 0 || nl > g.ncell[a] + 1) {
         ok = false;
       } else {
         l[a] = nl;
-        u[a] += k;
-        p[a] -= (double)k * g.hd[a];
+        rel[a] -= (double)k * g.hd[a];
       }
     }
   }
@@ -279,6 +329,8 @@ NZ_HD inline bool rebin(const Grid& g, double p[3], int l[3], int u[3]) {
 NZ_HD inline void keys_reset(const Grid& g, ScanKeys& k) {
   k.kin = 0u;
   k.kout = 0xffffffffu;
+  k.kin2 = 0u;
+  k.kout2 = 0xffffffffu;
   k.sid_new = (float)g.nstruct;
   k.gcnt = 0;
   k.gpos = 0u;
@@ -435,6 +487,11 @@ NZ_HD inline void keys_decode(const Grid& g, int cs, const ScanKeys& k, ScanAcc&
   const bool any_out = k.kout != 0xffffffffu;
   acc.fout = any_out ? u2f(k.kout & kKeyMask) : kLarge;
   acc.kout = any_out ? cand_sphere(g, cs + (int)(k.kout & 0xffu) - 1) : -1;
+  // the runners-up (sphere indices as for the best)
+  acc.fin2 = k.kin2 != 0u ? u2f(k.kin2 & kKeyMask) : -1.0f;
+  acc.kin2 = k.kin2 != 0u ? cand_sphere(g, cs + (int)(k.kin2 & 0xffu) - 1) : -1;
+  acc.fout2 = k.kout2 != 0xffffffffu ? u2f(k.kout2 & kKeyMask) : kLarge;
+  acc.kout2 = k.kout2 != 0xffffffffu ? cand_sphere(g, cs + (int)(k.kout2 & 0xffu) - 1) : -1;
   acc.exact = false;
 }
 
@@ -464,11 +521,9 @@ struct ExactGeom {
 
 NZ_HD NZ_EXACT_ATTR ScanAcc scan_exact_call(ExactGeom eg, int mode, int cs, int ce, double px,
                                            double py, double pz, double jx, double jy, double jz,
-                                           double lox, double loy, double loz, int lx, int ly,
-                                           int lz, float sid) {
+                                           int lx, int ly, int lz, float sid) {
   const double p[3] = {px, py, pz};
   const double j[3] = {jx, jy, jz};
-  const double lo[3] = {lox, loy, loz};
   Grid g;  // only the fields visit_exact and acc_reset read
   g.sph = eg.sph;
   g.sph_sid = eg.sph_sid;
@@ -476,7 +531,7 @@ NZ_HD NZ_EXACT_ATTR ScanAcc scan_exact_call(ExactGeom eg, int mode, int cs, int 
   ScanAcc acc;
   acc_reset(g, acc);
   if (mode == kExactAll) {
-    for (int64_t k = 0; k < eg.nspheres; ++k) visit_exact(g, p, j, lo, sid, (int)k, acc);
+    for (int64_t k = 0; k < eg.nspheres; ++k) visit_exact(g, p, j, sid, (int)k, acc);
     return acc;
   }
   const int r = (mode == kExactNeighbours) ? 1 : 0;
@@ -493,13 +548,13 @@ NZ_HD NZ_EXACT_ATTR ScanAcc scan_exact_call(ExactGeom eg, int mode, int cs, int 
           int k;
           const float w = eg.cand[2 * (int64_t)q + 1].w;
           __builtin_memcpy(&k, &w, sizeof(k));
-          visit_exact(g, p, j, lo, sid, k, acc);
+          visit_exact(g, p, j, sid, k, acc);
         }
       }
   return acc;
 }
 
-NZ_HD inline void scan_exact(const Grid& g, int mode, int cs, int ce, const double p[3],
+NZ_HD inline void scan_exact(const Grid& g, int mode, int cs, int ce, const double x[3],
                              const double j[3], const int l[3], float sid, ScanAcc& acc) {
   ExactGeom eg;
   eg.sph = g.sph;
@@ -510,8 +565,8 @@ NZ_HD inline void scan_exact(const Grid& g, int mode, int cs, int ce, const doub
   eg.nloc0 = g.nloc[0];
   eg.nloc1 = g.nloc[1];
   eg.nstruct = g.nstruct;
-  acc = scan_exact_call(eg, mode, cs, ce, p[0], p[1], p[2], j[0], j[1], j[2], cell_lo(g, 0, l[0]),
-                        cell_lo(g, 1, l[1]), cell_lo(g, 2, l[2]), l[0], l[1], l[2], sid);
+  acc = scan_exact_call(eg, mode, cs, ce, x[0], x[1], x[2], j[0], j[1], j[2], l[0], l[1], l[2],
+                        sid);
 }
 
 // Segment inside the union of the safety boxes of the 27 cells around the
@@ -530,9 +585,11 @@ NZ_HD inline bool in_neighbourhood(const Grid& g, const double p[3], const doubl
 
 // Resident per-spin state held in registers across the steps of a launch.
 struct Lane {
-  int u[3];      // unwrapped cell index
-  int l[3];      // local cell index (1..ncell inside the box, 0 / ncell+1 halo)
-  double p[3];   // position relative to the lower corner of cell l
+  double x[3];   // absolute position, wrapped at step ends as the reference wraps
+  int l[3];      // local index of the cell holding x (1..ncell inside the box,
+                 // 0 / ncell+1 halo)
+  int w[3];      // net +L shifts of the start position (the reference's spins0
+                 // follows every wrap: spins0 = start + w L)
   float logsig;  // log T2 weight
   float m[3];    // phase moment
   float sid;     // structure id after the last sub-step
@@ -540,13 +597,17 @@ struct Lane {
   int nguard;    // sub-steps whose scan ran in FP64
 };
 
-NZ_HD inline void lane_load_cells(const Grid& g, Lane& s) {
+// Position relative to the lower corner of the lane's cell (FP32 scan input
+// and the safety-box test; the FP64 decisions use x itself).
+NZ_HD inline void lane_rel(const Grid& g, const Lane& s, double rel[3]) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int w = s.u[a] % g.ncell[a];
-    if (w < 0) w += g.ncell[a];
-    s.l[a] = w + 1;
-  }
+  for (int a = 0; a < 3; ++a) rel[a] = s.x[a] - cell_lo(g, a, s.l[a]);
+}
+
+// The cell holding coordinate x on axis a, clamped to the grid's halo.
+NZ_HD inline int cell_of(const Grid& g, int a, double x) {
+  int c = (int)floor((x + 0.5 * g.Ld[a]) / g.hd[a]) + 1;
+  return c < 0 ? 0 : (c > g.ncell[a] + 1 ? g.ncell[a] + 1 : c);
 }
 
 NZ_HD inline void draw_jump(uint64_t seed, uint32_t spin, uint32_t step, uint32_t draw, double ds,
@@ -592,22 +653,39 @@ NZ_HD inline void step_draw(StepState& q, uint64_t seed, uint32_t spin, uint32_t
   q.pending = -1;
 }
 
+// x += jc j (sim3d_kernel.cu:460-462), each product rounded before the add.
+NZ_HD inline void move(Lane& s, const StepState& q, double jc) {
+#pragma clang fp contract(off)
+  s.x[0] += jc * q.j[0];
+  s.x[1] += jc * q.j[1];
+  s.x[2] += jc * q.j[2];
+}
+
+// Remaining fraction after a scatter, sqrt(max(0, jr^2 - jc^2)) (:470-471),
+// each square rounded before the difference.
+NZ_HD inline double scatter_remaining(double jr, double jc) {
+#pragma clang fp contract(off)
+  return sqrt(MAXd(0.0, jr * jr - jc * jc));
+}
+
 // One sub-step of the multi-collision loop (sim3d_kernel.cu:373-487).
 // Returns true when the step is complete (:373 loop condition).
 NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, uint32_t spin,
                           uint32_t step, double dt, bool do_t2) {
   q.niter++;
   double jc = q.jr;  // :376
+  double rel[3];
+  lane_rel(g, s, rel);
   ScanAcc acc;
   for (int pass = 0; pass < 3; ++pass) {
     int exact_mode = -1;
-    if (!in_safety(g, s.p, q.j, q.jr)) {
-      if (rebin(g, s.p, s.l, s.u)) {
+    if (!in_safety(g, rel, q.j, q.jr)) {
+      if (rebin(g, rel, s.l)) {
         const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
         q.cs = g.cell_start[cid];
         q.ce = g.cell_start[cid + 1];
-        if (!in_safety(g, s.p, q.j, q.jr))
-          exact_mode = in_neighbourhood(g, s.p, q.j, q.jr, s.l) ? kExactNeighbours : kExactAll;
+        if (!in_safety(g, rel, q.j, q.jr))
+          exact_mode = in_neighbourhood(g, rel, q.j, q.jr, s.l) ? kExactNeighbours : kExactAll;
       } else {
         exact_mode = kExactAll;
       }
@@ -617,7 +695,7 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
       ScanIn in;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        in.p[a] = (float)s.p[a];
+        in.p[a] = (float)rel[a];
         in.j[a] = (float)q.j[a];
       }
       const float a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
@@ -638,14 +716,13 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
         keys_decode(g, q.cs, k, acc);
         if (k.gcnt == 1) {  // the one undecided candidate in FP64
           s.nguard++;
-          const double lo[3] = {cell_lo(g, 0, s.l[0]), cell_lo(g, 1, s.l[1]), cell_lo(g, 2, s.l[2])};
-          visit_exact(g, s.p, q.j, lo, s.sid, cand_sphere(g, q.cs + (int)k.gpos - 1), acc);
+          visit_exact(g, s.x, q.j, s.sid, cand_sphere(g, q.cs + (int)k.gpos - 1), acc);
         }
       }
     }
     if (exact_mode >= 0) {  // rare: the scan in FP64
       s.nguard++;
-      scan_exact(g, exact_mode, q.cs, q.ce, s.p, q.j, s.l, s.sid, acc);
+      scan_exact(g, exact_mode, q.cs, q.ce, s.x, q.j, s.l, s.sid, acc);
     }
     if (acc.sid_new == s.sid) break;
     s.sid = acc.sid_new;  // structure changed: redo with the new id
@@ -655,43 +732,47 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
       draw_jump(seed, spin, step, 0u, q.ds, q.j);
     }
   }
-  // the decisive roots in FP64 (the oracle's arithmetic).  A root whose FP32
-  // value exceeds the smaller of jr and the other root by more than FP32
-  // round-off cannot decide jc = min(jr, fin, fout); it is not re-solved.
+  // The decisive roots in FP64 (the oracle's arithmetic).  A root whose FP32
+  // value exceeds the smaller of jr and the other kind's root by more than
+  // FP32 round-off cannot decide jc = min(jr, fin, fout) and is not
+  // re-solved; a runner-up within that round-off of the best of its kind is
+  // re-solved beside it (the FP32 keys cannot order the two).
   double fin = acc.any_in ? (double)acc.fin : (double)kLarge;
   double fout = (double)acc.fout;
   {
     const float fi = acc.any_in ? acc.fin : kLarge;
-    const float lim_in = fminf(acc.fout, (float)q.jr) * 1.0001f + 1e-6f;
-    const float lim_out = fminf(fi, (float)q.jr) * 1.0001f + 1e-6f;
+    const float lim_in = fminf(acc.fout, (float)q.jr) * (1.0f + kTieRel) + kTieAbs;
+    const float lim_out = fminf(fi, (float)q.jr) * (1.0f + kTieRel) + kTieAbs;
     const bool need_in = acc.kin >= 0 && acc.any_in && fi <= lim_in;
     const bool need_out = acc.kout >= 0 && acc.fout <= lim_out;
+    const bool need_in2 = need_in && acc.kin2 >= 0 && acc.fin2 >= fi * (1.0f - kTieRel) - kTieAbs;
+    const bool need_out2 =
+        need_out && acc.kout2 >= 0 && acc.fout2 <= acc.fout * (1.0f + kTieRel) + kTieAbs;
     if (acc.any_in && acc.kin < 0) fin = 0.0;
-    // A lane rarely needs both roots (a near-tie): the first pass solves the
-    // exit root or else the entry root, the second only the entry root of
-    // lanes that need both, so a wavefront usually runs the FP64 solve once.
+    // Usually one root per lane: the loop runs as often as the lane of the
+    // wavefront with the most near-ties needs.
+    int todo = (need_in ? 1 : 0) | (need_in2 ? 2 : 0) | (need_out ? 4 : 0) | (need_out2 ? 8 : 0);
 #pragma unroll 1
-    for (int r = 0; r < 2; ++r) {
-      const bool in_root = (r == 0) && need_in;
-      const bool want = (r == 0) ? (need_in || need_out) : (need_in && need_out);
-      if (want) {
-        const double4 c = g.sph[in_root ? acc.kin : acc.kout];
-        const double v = root_d(in_root, s.p[0] - (c.x - cell_lo(g, 0, s.l[0])),
-                                s.p[1] - (c.y - cell_lo(g, 1, s.l[1])),
-                                s.p[2] - (c.z - cell_lo(g, 2, s.l[2])), c.w, q.j[0], q.j[1], q.j[2]);
-        if (in_root) fin = v;
-        else fout = v;
-      }
+    while (todo != 0) {
+      const int bit = todo & -todo;
+      todo &= todo - 1;
+      const bool in_root = bit <= 2;
+      const int k = bit == 1 ? acc.kin : bit == 2 ? acc.kin2 : bit == 4 ? acc.kout : acc.kout2;
+      const double4 c = g.sph[k];
+      const double v = root_d(in_root, s.x[0] - c.x, s.x[1] - c.y, s.x[2] - c.z, c.w, q.j[0],
+                              q.j[1], q.j[2]);
+      if (bit == 1) fin = v;
+      else if (bit == 2) fin = MAXd(fin, v);
+      else if (bit == 4) fout = v;
+      else fout = MINd(fout, v);
     }
   }
   jc = MINd(jc, acc.any_in ? fin : (double)kLarge);  // :457
   jc = MINd(jc, fout);                               // :458
-  s.p[0] += jc * q.j[0];                             // :460-462
-  s.p[1] += jc * q.j[1];
-  s.p[2] += jc * q.j[2];
+  move(s, q, jc);                                    // :460-462
   const float inv_t2 = g.inv_T2[(int)s.sid];
   if (jc < q.jr) {  // scatter (:464-479)
-    q.jr = sqrt(MAXd(0.0, q.jr * q.jr - jc * jc));
+    q.jr = scatter_remaining(q.jr, jc);
     if (do_t2) s.logsig = fmaf((float)(-jc * jc * dt), inv_t2, s.logsig);
     s.nsub++;
     if (q.jr > (double)kTol && q.niter < kMaxIter)  // the draw is only used by a further sub-step
@@ -704,46 +785,34 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
 }
 
 // End of a time step: phase after the move and before the wrap (:674-676,
-// :880-887), re-base to the cell holding the end point, periodic wrap with the
-// phase correction (:489-521, :683-715, :891-942).
+// :880-887), the periodic wrap once per axis with the spins0 shift and the
+// phase correction (:489-521, :683-715, :891-942), then the cell holding the
+// end point.
 NZ_HD inline void step_end(const Grid& g, Lane& s, float G, float Ga) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double x = cell_lo(g, a, s.l[a]) + s.p[a];
-    s.m[a] = fmaf(G, (float)x, s.m[a]);
-    // the cell holding the end point: k cells from l (r = p - k h in [0, h));
-    // a step rarely leaves the neighbouring cell, so the division is a
-    // fallback and the usual case is two comparisons
-    double r = s.p[a];
-    int k = 0;
-    if (r < 0.0) {
-      r += g.hd[a];
-      k = -1;
-    } else if (r >= g.hd[a]) {
-      r -= g.hd[a];
-      k = 1;
+    s.m[a] = fmaf(G, (float)s.x[a], s.m[a]);
+    const double La = g.Ld[a];
+    int l = s.l[a];
+    if (s.x[a] < -La / 2) {
+      s.x[a] += La;
+      s.w[a] += 1;
+      s.m[a] = fmaf(g.L[a], Ga, s.m[a]);
+      l += g.ncell[a];
+    } else if (s.x[a] > La / 2) {
+      s.x[a] -= La;
+      s.w[a] -= 1;
+      s.m[a] = fmaf(-g.L[a], Ga, s.m[a]);
+      l -= g.ncell[a];
     }
-    if (r < 0.0 || r >= g.hd[a]) {
-      k = (int)floor(s.p[a] / g.hd[a]);
-      r = s.p[a] - (double)k * g.hd[a];
-      if (r < 0.0) { r += g.hd[a]; --k; }
-      if (r >= g.hd[a]) { r -= g.hd[a]; ++k; }
-    }
-    int nl = s.l[a] + k;
-    if (nl == g.ncell[a] + 1 && r == 0.0) { --nl; --k; r = g.hd[a]; }  // x == L/2 stays
-    s.p[a] = r;
-    s.u[a] += k;
-    // wrap: x > L/2  <=>  nl > ncell;  x < -L/2  <=>  nl < 1.  w = net number
-    // of +L wraps (one unless the step spans more than the box); each one
-    // shifts the phase by -L*G_area
-    int w = 0;
-    if (nl > g.ncell[a]) w = (nl <= 2 * g.ncell[a]) ? 1 : (nl - 1) / g.ncell[a];
-    else if (nl < 1) w = (nl > -g.ncell[a]) ? -1 : -((g.ncell[a] - nl) / g.ncell[a]);
-    if (w != 0) {
-      nl -= w * g.ncell[a];
-      s.m[a] = fmaf(-(float)w * g.L[a], Ga, s.m[a]);
-    }
-    s.l[a] = nl;
+    // a step rarely leaves the neighbouring cell: two comparisons, the
+    // division only as the fallback
+    const double r = s.x[a] - cell_lo(g, a, l);
+    if (r < 0.0) --l;
+    else if (r >= g.hd[a]) ++l;
+    const double r2 = s.x[a] - cell_lo(g, a, l);
+    if (l < 0 || l > g.ncell[a] + 1 || r2 < 0.0 || r2 >= g.hd[a]) l = cell_of(g, a, s.x[a]);
+    s.l[a] = l;
   }
 }
 
@@ -761,34 +830,33 @@ NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin
   step_end(g, s, G, Ga);
 }
 
-// Displacement since the start of the spin (unwrapped), per axis.
-NZ_HD inline double lane_disp(const Grid& g, const Lane& s, const int u0[3], const double p0[3],
-                              int a) {
-  return (double)(s.u[a] - u0[a]) * g.hd[a] + (s.p[a] - p0[a]);
+// Displacement since the start of the spin (unwrapped), per axis: position
+// minus the wrap-shifted start position, as the reference's x - spins0.
+NZ_HD inline double lane_disp(const Grid& g, const Lane& s, const double x0[3], int a) {
+#pragma clang fp contract(off)
+  return s.x[a] - (x0[a] + (double)s.w[a] * g.Ld[a]);
 }
 
-// Absolute position -> (cell, relative) with the structure id of the point.
+// Absolute position -> lane state with the structure id of the point.
 NZ_HD inline void lane_init(const Grid& g, const float pos[3], Lane& s) {
+#pragma clang fp contract(off)
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double x = (double)pos[a];
-    int c = (int)floor((x + 0.5 * g.Ld[a]) / g.hd[a]);
+    s.x[a] = (double)pos[a];
+    int c = (int)floor((s.x[a] + 0.5 * g.Ld[a]) / g.hd[a]);
     c = c < 0 ? 0 : (c > g.ncell[a] - 1 ? g.ncell[a] - 1 : c);
-    s.u[a] = c;
     s.l[a] = c + 1;
-    s.p[a] = x - cell_lo(g, a, c + 1);
+    s.w[a] = 0;
     s.m[a] = 0.0f;
   }
   // the cell list holds every sphere that can contain a point of the cell;
-  // decided in FP64 like the walk's guarded inside test
+  // decided in FP64 with the reference's inside test
   const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
   float sid = (float)g.nstruct;
   for (int k = g.cell_start[cid]; k < g.cell_start[cid + 1]; ++k) {
     const int idx = cand_sphere(g, k);
     const double4 c = g.sph[idx];
-    const double dx = s.p[0] - (c.x - cell_lo(g, 0, s.l[0]));
-    const double dy = s.p[1] - (c.y - cell_lo(g, 1, s.l[1]));
-    const double dz = s.p[2] - (c.z - cell_lo(g, 2, s.l[2]));
+    const double dx = s.x[0] - c.x, dy = s.x[1] - c.y, dz = s.x[2] - c.z;
     const float sc = (float)g.sph_sid[idx];
     if ((dx * dx + dy * dy + dz * dz - c.w * c.w) < 0 && sc < sid) sid = sc;
   }
@@ -800,14 +868,13 @@ NZ_HD inline void lane_init(const Grid& g, const float pos[3], Lane& s) {
 
 // Wrapped absolute position and the start position shifted by every wrap
 // (the reference's spins / spins0 after its wrap rule).
-NZ_HD inline void lane_outputs(const Grid& g, const Lane& s, const int u0[3], const double p0[3],
-                               float pos[3], float pos0[3]) {
+NZ_HD inline void lane_outputs(const Grid& g, const Lane& s, const double x0[3], float pos[3],
+                               float pos0[3]) {
+#pragma clang fp contract(off)
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int n = g.ncell[a];
-    const int wraps = (s.u[a] >= 0) ? s.u[a] / n : -((-s.u[a] + n - 1) / n);
-    pos[a] = (float)(cell_lo(g, a, s.l[a]) + s.p[a]);
-    pos0[a] = (float)(-0.5 * g.Ld[a] + (double)u0[a] * g.hd[a] + p0[a] - (double)wraps * g.Ld[a]);
+    pos[a] = (float)s.x[a];
+    pos0[a] = (float)(x0[a] + (double)s.w[a] * g.Ld[a]);
   }
 }
 

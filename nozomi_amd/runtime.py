@@ -25,8 +25,24 @@ def _stream_ptr(device):
     return _c_vp(torch.cuda.current_stream(device).cuda_stream)
 
 
-def _ptr(t):
-    return _c_vp(t.data_ptr()) if t is not None else _c_vp(0)
+def _ptr(t, device=None, name="tensor"):
+    """Device address of a tensor handed to a kernel (NULL for None).
+
+    The kernels dereference it as a dense array on the walker's GPU, so a
+    host tensor, a tensor on another GPU or a strided view is refused here,
+    before any launch: a host address would fault the GPU, and a view would be
+    read with the wrong layout."""
+    if t is None:
+        return _c_vp(0)
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch.Tensor on the GPU, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a GPU tensor (got one on {t.device})")
+    if device is not None and t.device != torch.device(device):
+        raise ValueError(f"{name} is on {t.device}, the walker on {torch.device(device)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous (call .contiguous())")
+    return _c_vp(t.data_ptr())
 
 
 class DeviceArray:
@@ -151,12 +167,15 @@ class DeviceSubstrate:
     def seed(self, n, targets=None, seed=0, max_attempts=4096, out=None):
         """Rejection-seed n spins into the target structures; returns [3, n]."""
         pos = out if out is not None else torch.empty((3, n), dtype=torch.float32, device=self.device)
+        if pos.dtype != torch.float32 or pos.numel() < 3 * n:
+            raise ValueError(f"out must be a float32 tensor with 3*{n} entries")
+        pp = _ptr(pos, self.device, "out")
         failed = torch.zeros(1, dtype=torch.int32, device=self.device)
         t = None if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
         with torch.cuda.device(self.device):
             check(lib().nz_seed(self.handle, n, None if t is None else t.ctypes.data_as(_c_vp),
                                 0 if t is None else t.size, int(seed) & (2**64 - 1),
-                                int(max_attempts), _ptr(pos), _ptr(failed),
+                                int(max_attempts), pp, _ptr(failed),
                                 _stream_ptr(self.device)), "nz_seed")
         nf = int(failed.item())
         if nf:
@@ -233,7 +252,10 @@ class Walker:
                                "the target structures are empty or too small")
 
     def walk(self, seed, nsteps, dt, g=None, garea=None, stepmom=None, t2=True):
-        """Advance nsteps (nz_walk).  g/garea: float32 device tensors [nsteps]."""
+        """Advance nsteps (nz_walk).  g/garea: float32 device tensors [nsteps];
+        stepmom: float64 device tensor [nsteps, 6] (sums are added)."""
+        if (g is None) != (garea is None):
+            raise ValueError("g and garea go together")
         if g is not None:
             if g.dtype != torch.float32 or garea.dtype != torch.float32:
                 raise ValueError("g and garea must be float32 tensors")
@@ -241,16 +263,29 @@ class Walker:
                 raise ValueError("g and garea need nsteps entries")
         if stepmom is not None and (stepmom.dtype != torch.float64 or stepmom.numel() < 6 * nsteps):
             raise ValueError("stepmom must be a float64 tensor with 6*nsteps entries")
-        with torch.cuda.device(self.device):
+        dev = self.device
+        pg, pa = _ptr(g, dev, "g"), _ptr(garea, dev, "garea")
+        pm = _ptr(stepmom, dev, "stepmom")
+        with torch.cuda.device(dev):
             check(lib().nz_walk(self.handle, int(seed) & (2**64 - 1), int(nsteps), float(dt),
-                                _ptr(g), _ptr(garea), _ptr(stepmom),
-                                _lib.NZ_WALK_T2 if t2 else 0, _stream_ptr(self.device)), "nz_walk")
+                                pg, pa, pm, _lib.NZ_WALK_T2 if t2 else 0, _stream_ptr(dev)),
+                  "nz_walk")
+
+    def _out(self, t, name, dtype, numel):
+        if t is None:
+            return _ptr(None)
+        if t.dtype != dtype or t.numel() < numel:
+            raise ValueError(f"{name} must be a {dtype} tensor with at least {numel} entries")
+        return _ptr(t, self.device, name)
 
     def read(self, pos=None, pos0=None, sig=None, mom=None):
         """Fill the given tensors (spin order): pos/pos0/mom [3, n], sig [n]."""
+        f32, n = torch.float32, self.n
+        args = (self._out(pos, "pos", f32, 3 * n), self._out(pos0, "pos0", f32, 3 * n),
+                self._out(sig, "sig", f32, n), self._out(mom, "mom", f32, 3 * n))
         with torch.cuda.device(self.device):
-            check(lib().nz_walker_read(self.handle, _ptr(pos), _ptr(pos0), _ptr(sig), _ptr(mom),
-                                       _stream_ptr(self.device)), "nz_walker_read")
+            check(lib().nz_walker_read(self.handle, *args, _stream_ptr(self.device)),
+                  "nz_walker_read")
 
     def read_f64(self):
         """FP64 wrapped positions and unwrapped displacements, numpy [3, n]."""
@@ -286,20 +321,37 @@ class Walker:
 
     def displacement_moments(self, out=None):
         out = out if out is not None else torch.empty(6, dtype=torch.float64, device=self.device)
+        po = self._out(out, "out", torch.float64, 6)
         with torch.cuda.device(self.device):
-            check(lib().nz_displacement_moments(self.handle, _ptr(out), _stream_ptr(self.device)),
+            check(lib().nz_displacement_moments(self.handle, po, _stream_ptr(self.device)),
                   "nz_displacement_moments")
         return out
+
+
+def _projection_args(mom, dirs, scale):
+    if mom.dim() != 2 or mom.shape[0] != 3 or mom.dtype != torch.float32:
+        raise ValueError("mom must be a float32 tensor of shape (3, n)")
+    if dirs.dim() != 2 or dirs.shape[0] != 3 or dirs.dtype != torch.float32:
+        raise ValueError("dirs must be a float32 tensor of shape (3, W)")
+    W = dirs.shape[1]
+    if scale is not None and (scale.dtype != torch.float32 or scale.numel() != W):
+        raise ValueError("scale must be a float32 tensor of W entries")
+    dev = mom.device
+    return (_ptr(mom, dev, "mom"), _ptr(dirs.contiguous(), dev, "dirs"),
+            _ptr(scale, dev, "scale"), dirs.contiguous())
 
 
 def project_phase(mom, dirs, scale=None, out=None):
     """phase[w, i] = scale[w] * dirs[:, w] . mom[:, i]  (nz_project_phase)."""
     n = mom.shape[1]
     W = dirs.shape[1]
-    dirs = dirs.contiguous()
+    pm, pd, ps, keep = _projection_args(mom, dirs, scale)
     out = out if out is not None else torch.empty((W, n), dtype=torch.float32, device=mom.device)
-    check(lib().nz_project_phase(n, W, _ptr(mom), _ptr(dirs), _ptr(scale), _ptr(out),
+    if out.dtype != torch.float32 or out.numel() < W * n:
+        raise ValueError("out must be a float32 tensor of W*n entries")
+    check(lib().nz_project_phase(n, W, pm, pd, ps, _ptr(out, mom.device, "out"),
                                  _stream_ptr(mom.device)), "nz_project_phase")
+    del keep
     return out
 
 
@@ -307,8 +359,11 @@ def signal_sums(mom, dirs, scale=None, out=None):
     """FP64 [W, 2] sums of (cos, sin) of the projected phases (nz_signal_reduce)."""
     n = mom.shape[1]
     W = dirs.shape[1]
-    dirs = dirs.contiguous()
+    pm, pd, ps, keep = _projection_args(mom, dirs, scale)
     out = out if out is not None else torch.empty((W, 2), dtype=torch.float64, device=mom.device)
-    check(lib().nz_signal_reduce(n, W, _ptr(mom), _ptr(dirs), _ptr(scale), _ptr(out),
+    if out.dtype != torch.float64 or out.numel() < 2 * W:
+        raise ValueError("out must be a float64 tensor of 2*W entries")
+    check(lib().nz_signal_reduce(n, W, pm, pd, ps, _ptr(out, mom.device, "out"),
                                  _stream_ptr(mom.device)), "nz_signal_reduce")
+    del keep
     return out

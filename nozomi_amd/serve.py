@@ -66,10 +66,11 @@ class Request:
 @dataclasses.dataclass
 class Result:
     rid: int
-    signals: np.ndarray      # complex [W] = sum exp(i phase) / n
+    signals: np.ndarray      # complex [W] = sum exp(i phase) / n (None on error)
     arrival: float
     admitted: float
     finished: float
+    error: str = None        # why the request failed (it never ran)
 
     @property
     def latency(self):
@@ -183,6 +184,14 @@ class Server:
             except BatchFull:
                 self.subs.release(req.key)
                 return admitted
+            except Exception as e:  # the request cannot run: answer it, keep serving
+                self.subs.release(req.key)
+                with self.lock:
+                    self.queue.popleft()
+                now = time.perf_counter()
+                self.results[req.rid] = Result(req.rid, None, req.arrival, now, now,
+                                               error=f"{type(e).__name__}: {e}")
+                continue
             with self.lock:
                 self.queue.popleft()
             self.proto_bytes += pb

@@ -39,8 +39,10 @@
 namespace {
 
 // ---------------------------------------------------------------- constants
-// sim3d_kernel.cu:34-35
-constexpr double kTol = 1e-4;
+// sim3d_kernel.cu:34-35: `const float tol = 1e-4;` and `someLargeNumber =
+// 1e6`, so the tolerance is the binary32 value nearest 1e-4
+// (9.99999974737875e-05), widened exactly to double here.
+constexpr double kTol = (double)1e-4f;
 constexpr double kLarge = 1e6;
 // sim3d_kernel.cu:373 (sub-step cap)
 constexpr int kMaxIter = 100;

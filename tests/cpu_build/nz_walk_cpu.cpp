@@ -22,8 +22,7 @@ using namespace nz;
 struct CpuWalker {
   HostGrid hg;
   std::vector<Lane> lanes;
-  std::vector<int> u0;
-  std::vector<double> p0;
+  std::vector<double> x0;
 };
 
 extern "C" {
@@ -68,9 +67,11 @@ void nzcpu_scan_compare(void* h, int n, const double* pos, const double* jump, c
     for (int a = 0; a < 3; ++a) q.j[a] = jump[a * n + i];
     const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
     const int cs = g.cell_start[cid], ce = g.cell_start[cid + 1];
+    double rel[3];
+    lane_rel(g, s, rel);
     ScanIn in;
     for (int a = 0; a < 3; ++a) {
-      in.p[a] = (float)s.p[a];
+      in.p[a] = (float)rel[a];
       in.j[a] = (float)q.j[a];
     }
     const float a2 = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
@@ -114,15 +115,11 @@ void nzcpu_info(void* h, double* out /* ncell[3], cell[3], margin, ncand_mean_bo
 void nzcpu_init(void* h, int N, const float* pos) {
   CpuWalker* w = (CpuWalker*)h;
   w->lanes.resize(N);
-  w->u0.resize(3 * (size_t)N);
-  w->p0.resize(3 * (size_t)N);
+  w->x0.resize(3 * (size_t)N);
   for (int i = 0; i < N; ++i) {
     const float p[3] = {pos[i], pos[N + i], pos[2 * N + i]};
     lane_init(w->hg.g, p, w->lanes[i]);
-    for (int a = 0; a < 3; ++a) {
-      w->u0[3 * i + a] = w->lanes[i].u[a];
-      w->p0[3 * i + a] = w->lanes[i].p[a];
-    }
+    for (int a = 0; a < 3; ++a) w->x0[3 * i + a] = w->lanes[i].x[a];
   }
 }
 
@@ -141,7 +138,7 @@ void nzcpu_walk(void* h, uint64_t seed, int64_t step0, int nsteps, double dt, co
       walk_step(w->hg.g, s, seed, (uint32_t)i, (uint32_t)(step0 + t), dt, do_t2 != 0, G, Ga);
     }
     float pp[3], pp0[3];
-    lane_outputs(w->hg.g, s, &w->u0[3 * i], &w->p0[3 * i], pp, pp0);
+    lane_outputs(w->hg.g, s, &w->x0[3 * i], pp, pp0);
     for (int a = 0; a < 3; ++a) {
       pos[a * N + i] = pp[a];
       pos0[a * N + i] = pp0[a];
@@ -173,8 +170,8 @@ void nzcpu_state_f64(void* h, double* pos, double* disp) {
   for (int i = 0; i < N; ++i) {
     const Lane& s = w->lanes[i];
     for (int a = 0; a < 3; ++a) {
-      pos[a * N + i] = cell_lo(g, a, s.l[a]) + s.p[a];
-      disp[a * N + i] = lane_disp(g, s, &w->u0[3 * i], &w->p0[3 * i], a);
+      pos[a * N + i] = s.x[a];
+      disp[a * N + i] = lane_disp(g, s, &w->x0[3 * i], a);
     }
   }
 }

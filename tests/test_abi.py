@@ -49,3 +49,24 @@ def test_argument_errors_are_reported():
     rc = L.nz_batch_add(None, None, 10, 0, 10, 0.01, None, None, 0, 64, None, None, 0,
                         ctypes.byref(sim), None)
     assert rc < 0 and b"batch" in L.nz_last_error_string()
+
+
+def test_host_tensors_are_refused_before_any_launch():
+    """A host (CPU) tensor, or a strided view, handed where a kernel expects
+    device memory raises ValueError in the Python wrapper: the library is
+    never called with a host address (that would fault the GPU)."""
+    import pytest
+    import torch
+    from nozomi_amd import runtime as rt
+    with pytest.raises(ValueError, match="GPU tensor"):
+        rt._ptr(torch.zeros(6, dtype=torch.float64), None, "stepmom")
+    with pytest.raises(ValueError, match="torch.Tensor"):
+        rt._ptr([0.0] * 6, None, "stepmom")
+    mom = torch.zeros((3, 16), dtype=torch.float32)
+    dirs = torch.zeros((3, 4), dtype=torch.float32)
+    with pytest.raises(ValueError, match="GPU tensor"):
+        rt.signal_sums(mom, dirs)
+    with pytest.raises(ValueError, match="GPU tensor"):
+        rt.project_phase(mom, dirs)
+    with pytest.raises(ValueError, match="shape"):
+        rt.signal_sums(mom.T, dirs)

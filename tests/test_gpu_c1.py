@@ -1,0 +1,158 @@
+"""The headline configuration on the GPU: 200 beaded axons in a 60 um
+periodic cube (~7e4 spheres with images), D = 2.0 um^2/ms, dt = 10 us, PGSE
+delta = 10 ms, Delta = 40 ms at b = 1, 2, 3 ms/um^2 with 64
+Fibonacci-hemisphere directions per shell (192 signals).
+
+1. Trajectory parity: 2e4 spins, 1000 steps, against the FP64 oracle on the
+   same start positions and Philox stream (randomWalk3d_phase_multidirr,
+   sim3d_kernel.cu:719-943, as restated by the oracle).
+2. The full 1e6-spin, 5000-step simulation, checked by properties that do
+   not depend on the size: wrapped positions stay in the box, impermeable
+   membranes (a spin stays in the axon, or the extra-axonal space, it
+   started in), the signal of
+   every direction falls with b and stays above the free-diffusion value
+   exp(-b D0), the imaginary parts vanish, and the device FP64 reduction
+   equals a host FP64 sum over the same moments.
+"""
+import numpy as np
+import pytest
+import torch
+from scipy.spatial import cKDTree
+
+from nozomi_amd import runtime as rt
+from nozomi_amd.protocol import PGSEShells, fibonacci_hemisphere
+from nozomi_amd.substrate import generate_beaded_axons, load_geometry
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261203
+
+
+@pytest.fixture(scope="module")
+def c1():
+    M, L = generate_beaded_axons(200, 60.0, seed=SEED)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    prot = PGSEShells(10.0, 40.0, 0.01, [1.0, 2.0, 3.0], fibonacci_hemisphere(64))
+    sub = geom.device_substrate(device=torch.device("cuda:0"))
+    return geom, prot, sub
+
+
+def structure_of(geom, pts):
+    """Smallest structure id of a sphere containing each point (FP64, strict
+    inside test as sim3d_kernel.cu:72-75); the box id when none does."""
+    sp = geom.spheres
+    tree = cKDTree(sp[:3].T)
+    rmax = float(sp[3].max())
+    out = np.full(pts.shape[1], geom.nstructures, dtype=np.int64)
+    for i, cand in enumerate(tree.query_ball_point(pts.T, rmax)):
+        if not cand:
+            continue
+        c = np.asarray(cand)
+        d = pts[:, i][:, None] - sp[:3, c]
+        inside = (d * d).sum(0) - sp[3, c] ** 2 < 0
+        if inside.any():
+            out[i] = int(sp[4, c[inside]].min())
+    return out
+
+
+def test_c1_trajectory_parity_1000_steps(c1, oracle_lib):
+    geom, prot, sub = c1
+    N, T = 20000, 1000
+    print(f"c1 parity: {N} spins x {T} steps", flush=True)
+    w = rt.Walker(sub, N, counters=True)
+    start = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=SEED + 1)
+    w.set_positions(start)
+    g = torch.from_numpy(prot.g[:T].copy()).cuda()
+    ga = torch.from_numpy(prot.garea[:T].copy()).cuda()
+    w.walk(SEED, T, prot.dt, g=g, garea=ga, t2=False)
+    gp, gd = w.read_f64()
+    mom = torch.empty((3, N), dtype=torch.float32, device="cuda:0")
+    w.read(mom=mom)
+    mom = mom.cpu().numpy().astype(np.float64)
+    p0 = start.cpu().numpy().astype(np.float64)
+    orc = oracle_lib.OracleSubstrate(geom.spheres, geom.nstructures, geom.structure_D(),
+                                     geom.structure_T2(), (geom.Lx, geom.Ly, geom.Lz))
+    pos, pos0, sig = p0.copy(), p0.copy(), np.ones(N)
+    omom = np.zeros((3, N))
+    nsub_o = np.zeros(N, dtype=np.int64)
+    orc.walk(pos, pos0, sig, seed=SEED, step0=0, nsteps=T, dt=prot.dt,
+             g=prot.g[:T].astype(np.float64), garea=prot.garea[:T].astype(np.float64), mom=omom,
+             nsub=nsub_o)
+    L = geom.Lx
+    d = gp - pos
+    err = np.maximum(np.abs(d - L * np.round(d / L)).max(axis=0),
+                     np.abs(gd - (pos - pos0)).max(axis=0))
+    few = nsub_o <= 60
+    exact = np.mean(err == 0.0)
+    print(f"  within 1e-5 um: all {np.mean(err < 1e-5):.4f}, <=60 collisions "
+          f"{np.mean(err[few] < 1e-5):.4f} ({few.sum()} spins); bit-identical {exact:.4f}; "
+          f"median {np.median(err):.2e}; collisions/spin {nsub_o.mean():.1f}", flush=True)
+    assert np.mean(err[few] < 1e-5) > 0.995
+    assert np.mean(err < 1e-5) > 0.80
+    # moments of the agreeing spins: FP32 accumulation of 1000 terms
+    ok = err < 1e-5
+    scale = np.abs(prot.g[:T]).sum() * L
+    assert np.abs(mom[:, ok] - omom[:, ok]).max() < 1e-5 * scale
+
+
+def test_c1_full_run_properties(c1):
+    geom, prot, sub = c1
+    N, T = 1_000_000, prot.nsteps
+    assert T == 5000
+    print(f"c1 full run: {N} spins x {T} steps", flush=True)
+    w = rt.Walker(sub, N)
+    start = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=SEED + 1)
+    w.set_positions(start)
+    g = torch.from_numpy(prot.g.copy()).cuda()
+    ga = torch.from_numpy(prot.garea.copy()).cuda()
+    w.walk(SEED, T, prot.dt, g=g, garea=ga, t2=False)
+    assert w.step == T
+    pos = torch.empty((3, N), dtype=torch.float32, device="cuda:0")
+    mom = torch.empty((3, N), dtype=torch.float32, device="cuda:0")
+    w.read(pos=pos, mom=mom)
+    dirs = torch.from_numpy(prot.dirs.astype(np.float32)).cuda()
+    scales = torch.from_numpy(prot.scales.astype(np.float32)).cuda()
+    sums = rt.signal_sums(mom, dirs, scales).cpu().numpy()
+    S = (sums[:, 0] + 1j * sums[:, 1]) / N
+    p = pos.cpu().numpy()
+    m = mom.cpu().numpy()
+    assert np.isfinite(p).all() and np.isfinite(m).all() and np.isfinite(S).all()
+    half = np.array([geom.Lx, geom.Ly, geom.Lz])[:, None] / 2
+    assert (np.abs(p) <= half).all()
+    # impermeable membranes: every spin of a subsample ends in its start structure
+    sub_i = np.random.default_rng(5).choice(N, 50000, replace=False)
+    p0 = start.cpu().numpy()[:, sub_i].astype(np.float64)
+    s0 = structure_of(geom, p0)
+    s1 = structure_of(geom, p[:, sub_i].astype(np.float64))
+    intra = s0 < geom.nstructures
+    print(f"  intra fraction {intra.mean():.3f}; structure id changes {int((s0 != s1).sum())}",
+          flush=True)
+    assert 0.2 < intra.mean() < 0.8
+    # the xy periodic image of an axon is a structure of its own (the
+    # reference splits the substrate matrix into fibers at every z = L/2
+    # marker), so a spin that wraps across an x or y face inside an axon
+    # moves to its image: compare axons by their xy centre modulo the box
+    L = geom.Lx
+    fam = np.full(geom.nstructures + 1, -1, dtype=np.int64)
+    sid = geom.spheres[4].astype(np.int64)
+    first = np.unique(sid, return_index=True)[1]
+    cxy = np.round(np.mod(geom.spheres[:2, first] + L / 2, L) * 1e4).astype(np.int64)
+    fam[sid[first]] = cxy[0] * 10**9 + cxy[1]
+    same = np.where(intra, fam[s0] == fam[s1], s1 == geom.nstructures)
+    assert np.mean(same) > 0.9999
+    # signals: 3 shells x 64 directions
+    Sr = S.real.reshape(3, 64)
+    b = np.array([1.0, 2.0, 3.0])[:, None]
+    tol = 4.0 / np.sqrt(2 * N)
+    print(f"  mean signal per shell {Sr.mean(axis=1)}; max |Im| {np.abs(S.imag).max():.2e}",
+          flush=True)
+    assert np.abs(S.imag).max() < tol
+    assert (Sr[0] - Sr[1] > -tol).all() and (Sr[1] - Sr[2] > -tol).all()
+    assert (Sr > np.exp(-b * 2.0) - tol).all()
+    assert (Sr < 1.0).all()
+    # device FP64 reduction against a host FP64 sum over the same moments
+    k = 4096
+    ph = (prot.scales[:, None] * (prot.dirs.T @ m[:, :k].astype(np.float64)))
+    host = np.exp(1j * ph.astype(np.float32).astype(np.float64)).sum(axis=1)
+    dev = rt.signal_sums(mom[:, :k].contiguous(), dirs, scales).cpu().numpy()
+    assert np.abs(dev[:, 0] + 1j * dev[:, 1] - host).max() < 1e-3

@@ -4,7 +4,7 @@ root finding."""
 import numpy as np
 import pytest
 
-TOL = 1e-4
+TOL = float(np.float32(1e-4))  # sim3d_kernel.cu:34, `const float tol = 1e-4`
 
 
 def test_inside_axis_aligned(oracle_lib):

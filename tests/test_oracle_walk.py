@@ -16,7 +16,7 @@ import pytest
 from nozomi_amd.substrate import single_sphere_geometry
 from oracle.reference_segments import reference_segments
 
-TOL = 1e-4
+TOL = float(np.float32(1e-4))  # sim3d_kernel.cu:34, `const float tol = 1e-4`
 
 
 def _arena(oracle_lib, spheres, L, D=2.0, T2=100.0):
@@ -60,6 +60,51 @@ def test_free_walk_known_answer(oracle_lib):
     assert np.abs(np.round((x0 + disp) / L)).sum() > 10
     assert np.allclose(mom, mom_ref, atol=1e-10, rtol=0)
     assert np.allclose(sig, np.exp(-T * dt / 80.0), rtol=1e-12)
+
+
+def test_step_moments_known_answer(oracle_lib):
+    """Per-step moment sums (computeDiffusionCoefficientsAndKurtosis,
+    sim3d_kernel.cu:162-238, restated at walker_oracle.cpp's stepmom) of free
+    spins in a small periodic box: row t holds sum_i d_i(t)^2 per axis, then
+    sum_i d_i(t)^4, with d the unwrapped displacement after step t's wrap,
+    built here from the RNG known answer.  A wrong step index, axis order,
+    power, or a displacement taken before the spins0 shift fails it."""
+    L, D, dt, T, N, seed = 1.5, 2.0, 0.01, 120, 24, 4321
+    o = _arena(oracle_lib, _far_sphere(L), L, D=D)
+    x0 = np.zeros((3, N))
+    x0[1] = np.linspace(-0.3, 0.3, N)
+    pos, pos0, sig = x0.copy(), x0.copy(), np.ones(N)
+    res = o.walk(pos, pos0, sig, seed=seed, step0=0, nsteps=T, dt=dt, stepmom=True)
+    ds = np.sqrt(2 * D * dt)
+    traj = np.zeros((T, 3, N))
+    for i in range(N):
+        steps = np.stack([oracle_lib.normals(seed, 0, i, t, 0, 1)[0] for t in range(T)])
+        traj[:, :, i] = np.cumsum(ds * steps, axis=0)
+    ref = np.concatenate([(traj ** 2).sum(2), (traj ** 4).sum(2)], axis=1)  # [T, 6]
+    assert np.abs(np.round((x0 + traj[-1]) / L)).sum() > 5  # wraps happened
+    assert np.allclose(res["stepmom"], ref, rtol=1e-12, atol=1e-14)
+
+
+def test_step_moments_free_diffusion_closed_form(oracle_lib):
+    """Empty arena, N spins: E[sum d^2] = N 2 D t and E[sum d^4] = 3 N (2 D t)^2
+    per axis at t = (k+1) dt (Gaussian displacements), within 5 standard
+    errors (var d^2 = 2 s^4, var d^4 = 96 s^8); the excess kurtosis of the
+    reference's host formula K = N S4 / S2^2 - 3 is then near 0."""
+    L, D, dt, T, N = 40.0, 2.0, 0.01, 200, 20000
+    o = _arena(oracle_lib, _far_sphere(L), L, D=D)
+    rng = np.random.default_rng(3)
+    x0 = (rng.random((3, N)) - 0.5) * 4.0
+    pos, pos0, sig = x0.copy(), x0.copy(), np.ones(N)
+    res = o.walk(pos, pos0, sig, seed=99, step0=0, nsteps=T, dt=dt, stepmom=True)
+    sm = res["stepmom"]
+    s2 = 2 * D * dt * np.arange(1, T + 1)  # variance per axis after each step
+    for a in range(3):
+        z2 = (sm[:, a] - N * s2) / (np.sqrt(2 * N) * s2)
+        z4 = (sm[:, 3 + a] - 3 * N * s2 ** 2) / (np.sqrt(96 * N) * s2 ** 2)
+        assert np.abs(z2).max() < 5, np.abs(z2).max()
+        assert np.abs(z4).max() < 5, np.abs(z4).max()
+    K = N * sm[-1, 3:] / sm[-1, :3] ** 2 - 3
+    assert np.abs(K).max() < 0.15, K
 
 
 def _first_jump(oracle_lib, seed, spin, step, draw, ds):
