@@ -80,9 +80,15 @@ int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
 // Sum over the n entries of mom [3, ld] (entries with valid[i] < 0 skipped)
-// of cos / sin of scale[w] * dirs[:, w] . m, into out [2W] (FP64).
+// of cos / sin of scale[w] * dirs[:, w] . m, into out [2W] (FP64): per-block
+// FP64 partials into partial [signal_partial_len(n, W)], then summed over the
+// blocks in block order.  _pooled takes the partials from the stream's pool.
+int64_t signal_partial_len(int64_t n, int32_t W);
 int signal_reduce(int64_t n, int64_t ld, const int32_t* valid, int32_t W, const float* mom,
-                  const float* dirs, const float* scale, double* out, hipStream_t stream);
+                  const float* dirs, const float* scale, double* partial, double* out,
+                  hipStream_t stream);
+int signal_reduce_pooled(int64_t n, int64_t ld, const int32_t* valid, int32_t W, const float* mom,
+                         const float* dirs, const float* scale, double* out, hipStream_t stream);
 
 // In-place exclusive scan of counts[0, m) (int32; the total must fit), with
 // a work array of scan_partial_len(m) ints; launches on `stream`.

@@ -7,7 +7,8 @@
 // projection of the wrap corrections): the walker keeps that 3-vector moment
 // and the W phases are formed once, here.  The signal step (sum over spins of
 // exp(i*phase)) has no counterpart in the reference, whose DWI caller is
-// missing; the sums are accumulated in FP64.
+// missing; the sums are accumulated in FP64, per block and then over the
+// blocks in a fixed order, so a given input always gives the same bits.
 #include <hip/hip_runtime.h>
 
 #include "nz_internal.h"
@@ -25,18 +26,29 @@ __global__ void __launch_bounds__(256) project_kernel(int64_t n, int32_t W, cons
   phase[(int64_t)w * n + i] = s * (ux * mom[i] + uy * mom[n + i] + uz * mom[2 * n + i]);
 }
 
-constexpr int kSigTile = 256;   // spins staged per LDS tile
-constexpr int kSigTiles = 8;    // tiles per block
+constexpr int kSigTile = 256;       // spins staged per LDS tile
+constexpr int kSigMaxBlocks = 1024; // blocks, i.e. FP64 partial sums per signal
 
-// Block: kSigTiles*kSigTile spins.  Thread t handles signals t, t+256, ...
-// reading every staged moment from LDS (broadcast reads).  mom is [3, ld];
-// entries whose valid[i] < 0 (padding slots of a batch) count as absent.
+// Blocks of a reduction over n spins and the tiles each walks: the partition
+// depends on n only, so a given n is always summed in the same order.
+static void signal_partition(int64_t n, int64_t* nb, int64_t* tiles_per_block) {
+  const int64_t ntiles = (n + kSigTile - 1) / kSigTile;
+  *nb = ntiles < kSigMaxBlocks ? ntiles : kSigMaxBlocks;
+  *tiles_per_block = (ntiles + *nb - 1) / *nb;
+}
+
+// Block b sums tiles [b tpb, (b + 1) tpb) of the spins.  Thread t handles
+// signals t, t+256, ... reading every staged moment from LDS (broadcast
+// reads).  mom is [3, ld]; entries whose valid[i] < 0 (padding slots of a
+// batch) count as absent.  The block's FP64 sums go to partial[b][2W]
+// (cos, sin interleaved), every entry written.
 __global__ void __launch_bounds__(256) signal_kernel(int64_t n, int64_t ld, const int32_t* valid,
                                                      int32_t W, const float* mom,
                                                      const float* dirs, const float* scale,
-                                                     double* out) {
+                                                     int64_t tiles_per_block, double* partial) {
   __shared__ float sm[3][kSigTile];
-  const int64_t base = (int64_t)blockIdx.x * kSigTile * kSigTiles;
+  __shared__ int wcnt[kSigTile / 64];
+  const int64_t base = (int64_t)blockIdx.x * tiles_per_block * kSigTile;
   constexpr int kMaxPer = 8;  // signals per thread (W <= 2048)
   double ac[kMaxPer], as[kMaxPer];
   float ux[kMaxPer], uy[kMaxPer], uz[kMaxPer];
@@ -50,8 +62,8 @@ __global__ void __launch_bounds__(256) signal_kernel(int64_t n, int64_t ld, cons
     uy[q] = (w < W) ? s * dirs[W + w] : 0.0f;
     uz[q] = (w < W) ? s * dirs[2 * W + w] : 0.0f;
   }
-  for (int tile = 0; tile < kSigTiles; ++tile) {
-    const int64_t t0 = base + (int64_t)tile * kSigTile;
+  for (int64_t tile = 0; tile < tiles_per_block; ++tile) {
+    const int64_t t0 = base + tile * kSigTile;
     if (t0 >= n) break;
     const int64_t i = t0 + threadIdx.x;
     __syncthreads();
@@ -62,7 +74,6 @@ __global__ void __launch_bounds__(256) signal_kernel(int64_t n, int64_t ld, cons
     // the present entries are packed to the front of the tile (ballot + count)
     const unsigned long long m = __ballot(ok);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __shared__ int wcnt[kSigTile / 64];
     if (lane == 0) wcnt[wave] = __popcll(m);
     __syncthreads();
     int before = 0, cnt = 0;
@@ -98,32 +109,60 @@ __global__ void __launch_bounds__(256) signal_kernel(int64_t n, int64_t ld, cons
       as[q] += (double)ps;
     }
   }
+  double* out = partial + (int64_t)blockIdx.x * 2 * W;
 #pragma unroll
   for (int q = 0; q < kMaxPer; ++q) {
     const int w = threadIdx.x + q * 256;
     if (w < W) {
-      atomicAdd(&out[2 * w], ac[q]);
-      atomicAdd(&out[2 * w + 1], as[q]);
+      out[2 * w] = ac[q];
+      out[2 * w + 1] = as[q];
     }
   }
 }
 
-__global__ void zero_f64(double* p, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
+// out[e] = sum over the nb blocks of partial[b][e], in block order (2W threads).
+__global__ void __launch_bounds__(256) signal_sum_kernel(const double* partial, int64_t nb,
+                                                         int32_t W, double* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2 * (int64_t)W) return;
+  double s = 0.0;
+  for (int64_t b = 0; b < nb; ++b) s += partial[b * 2 * W + e];
+  out[e] = s;
+}
+
+int64_t signal_partial_len(int64_t n, int32_t W) {
+  int64_t nb, tpb;
+  signal_partition(n, &nb, &tpb);
+  return nb * 2 * (int64_t)W;
 }
 
 int signal_reduce(int64_t n, int64_t ld, const int32_t* valid, int32_t W, const float* mom,
-                  const float* dirs, const float* scale, double* out, hipStream_t stream) {
+                  const float* dirs, const float* scale, double* partial, double* out,
+                  hipStream_t stream) {
   if (n <= 0 || W <= 0 || W > 2048) return fail(-1, "signal reduction: need n > 0 and 0 < W <= 2048");
-  if (!mom || !dirs || !out) return fail(-1, "signal reduction: NULL buffer");
-  // a fill kernel rather than hipMemsetAsync, so the call can be graph-captured
-  hipLaunchKernelGGL(zero_f64, dim3((unsigned)((2 * W + 255) / 256)), dim3(256), 0, stream, out,
-                     (int64_t)2 * W);
-  const int64_t per = (int64_t)kSigTile * kSigTiles;
-  hipLaunchKernelGGL(signal_kernel, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0, stream, n,
-                     ld, valid, W, mom, dirs, scale, out);
+  if (!mom || !dirs || !out || !partial) return fail(-1, "signal reduction: NULL buffer");
+  int64_t nb, tpb;
+  signal_partition(n, &nb, &tpb);
+  hipLaunchKernelGGL(signal_kernel, dim3((unsigned)nb), dim3(256), 0, stream, n, ld, valid, W, mom,
+                     dirs, scale, tpb, partial);
+  hipLaunchKernelGGL(signal_sum_kernel, dim3((unsigned)((2 * W + 255) / 256)), dim3(256), 0, stream,
+                     (const double*)partial, nb, W, out);
   NZ_HIP(hipGetLastError());
+  return 0;
+}
+
+// Signal reduction with library-allocated partials, taken from the stream's
+// memory pool and returned to it after the two kernels (stream-ordered: no
+// synchronisation).
+int signal_reduce_pooled(int64_t n, int64_t ld, const int32_t* valid, int32_t W, const float* mom,
+                         const float* dirs, const float* scale, double* out, hipStream_t stream) {
+  if (n <= 0 || W <= 0 || W > 2048) return fail(-1, "signal reduction: need n > 0 and 0 < W <= 2048");
+  void* partial = nullptr;
+  NZ_HIP(hipMallocAsync(&partial, sizeof(double) * (size_t)signal_partial_len(n, W), stream));
+  const int rc = signal_reduce(n, ld, valid, W, mom, dirs, scale, (double*)partial, out, stream);
+  const hipError_t e = hipFreeAsync(partial, stream);
+  if (rc != 0) return rc;
+  NZ_HIP(e);
   return 0;
 }
 
@@ -147,7 +186,7 @@ int nz_project_phase(int64_t n, int32_t W, const float* mom, const float* dirs, 
 int nz_signal_reduce(int64_t n, int32_t W, const float* mom, const float* dirs, const float* scale,
                      double* out, void* stream) {
   if (n <= 0 || W <= 0 || W > 2048) return fail(-1, "nz_signal_reduce: need n > 0 and 0 < W <= 2048");
-  return signal_reduce(n, n, nullptr, W, mom, dirs, scale, out, (hipStream_t)stream);
+  return signal_reduce_pooled(n, n, nullptr, W, mom, dirs, scale, out, (hipStream_t)stream);
 }
 
 }  // extern "C"

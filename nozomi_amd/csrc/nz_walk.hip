@@ -1507,8 +1507,8 @@ int nz_batch_signals(nz_batch_t bb, int32_t sim, int32_t W, const float* dirs, c
   NZ_HIP(hipEventRecord(b->ev_out, b->stream));
   NZ_HIP(hipStreamWaitEvent((hipStream_t)stream, b->ev_out, 0));
   const SlotState v = batch_view(b, sim);
-  return signal_reduce(j.nb * kBlk, b->cap, v.spin_id, W, v.mom, dirs, scale, out,
-                       (hipStream_t)stream);
+  return signal_reduce_pooled(j.nb * kBlk, b->cap, v.spin_id, W, v.mom, dirs, scale, out,
+                              (hipStream_t)stream);
 }
 
 }  // extern "C"
