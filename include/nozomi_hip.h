@@ -264,6 +264,18 @@ int nz_project_phase(int64_t n, int32_t W, const float* mom, const float* dirs,
 int nz_signal_reduce(int64_t n, int32_t W, const float* mom, const float* dirs,
                      const float* scale, double* out, void* stream);
 
+/* The same signal reduction without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-block FP64 partials live in a static device array of
+ * 1024 x 2 x 512 doubles (8 MiB), one copy per GPU, held until the process
+ * ends; W is limited to 512 (other sizes return < 0 with a message).  The
+ * result equals nz_signal_reduce's bit for bit.  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller. */
+int nz_signal_reduce_noalloc(int64_t n, int32_t W, const float* mom, const float* dirs,
+                             const float* scale, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

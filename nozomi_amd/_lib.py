@@ -102,6 +102,8 @@ _SIGS = {
                                         _c_vp, _c_vp]),
     "nz_signal_reduce": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_vp, _c_vp, _c_vp,
                                         _c_vp, _c_vp]),
+    "nz_signal_reduce_noalloc": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_vp, _c_vp,
+                                                _c_vp, _c_vp, _c_vp]),
 }
 
 EXPORTED = tuple(_SIGS)

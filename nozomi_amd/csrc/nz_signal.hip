@@ -11,6 +11,8 @@
 // blocks in a fixed order, so a given input always gives the same bits.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "nz_internal.h"
 
 namespace nz {
@@ -166,6 +168,38 @@ int signal_reduce_pooled(int64_t n, int64_t ld, const int32_t* valid, int32_t W,
   return 0;
 }
 
+// ---- allocation-free variant ------------------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops, harnesses that forbid it): the partials live
+// in a file-scope device array, one copy per GPU, held until the process
+// ends.  Its host-side name is not a device address: the address of each
+// GPU's copy is looked up once per device (hipGetSymbolAddress, with that
+// device current) and only that pointer is handed to the kernels.
+constexpr int32_t kNoallocMaxW = 512;
+constexpr int64_t kNoallocPartials = (int64_t)kSigMaxBlocks * 2 * kNoallocMaxW;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoallocPartials];
+
+constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// This device's copy of `scratch` (looked up on the first call on a device).
+static int scratch_of_current_device(double** out) {
+  int dev = 0;
+  NZ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(-1, "nz_signal_reduce_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    NZ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(-1, "nz_signal_reduce_noalloc: no device address for the scratch array");
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
 }  // namespace nz
 
 using namespace nz;
@@ -187,6 +221,19 @@ int nz_signal_reduce(int64_t n, int32_t W, const float* mom, const float* dirs, 
                      double* out, void* stream) {
   if (n <= 0 || W <= 0 || W > 2048) return fail(-1, "nz_signal_reduce: need n > 0 and 0 < W <= 2048");
   return signal_reduce_pooled(n, n, nullptr, W, mom, dirs, scale, out, (hipStream_t)stream);
+}
+
+int nz_signal_reduce_noalloc(int64_t n, int32_t W, const float* mom, const float* dirs,
+                             const float* scale, double* out, void* stream) {
+  if (n <= 0 || W <= 0 || W > kNoallocMaxW)
+    return fail(-1, "nz_signal_reduce_noalloc: need n > 0 and 0 < W <= 512 (the static scratch "
+                    "holds 1024 blocks x 2W FP64 partials)");
+  if (!mom || !dirs || !out) return fail(-1, "nz_signal_reduce_noalloc: NULL buffer");
+  if (signal_partial_len(n, W) > kNoallocPartials)
+    return fail(-1, "nz_signal_reduce_noalloc: partials exceed the static scratch");
+  double* partial = nullptr;
+  if (int rc = scratch_of_current_device(&partial)) return rc;
+  return signal_reduce(n, n, nullptr, W, mom, dirs, scale, partial, out, (hipStream_t)stream);
 }
 
 }  // extern "C"
