@@ -1,6 +1,7 @@
 """The narrow-pulse ADC driver and the DWI gradient driver end to end on a
 synthetic substrate pickle, against the FP64 oracle walked from the same
-start positions with the same Philox key."""
+start positions with the same Philox key (trajectories are the oracle's, so
+the bars are round-off bars)."""
 import os
 import pickle
 
@@ -44,7 +45,9 @@ def test_narrow_pulse_driver_matches_oracle(substrate_file, oracle_lib, compartm
     _, D_ref, _ = drivers.adc_from_step_moments(out["stepmom"], p.shape[1], 0.02)
     rel = np.abs(table[:, :3] / D_ref - 1)
     print(f"ADC vs oracle: max rel {rel.max():.2e}")
-    assert rel.max() < 1e-2
+    # equal trajectories and FP64 moment sums: only the summation order and
+    # the table's storage precision separate the two
+    assert rel.max() < 1e-6
     if compartment == "intra":
         # restricted: the transverse ADC falls well below D0 within 4 ms
         assert table[-1, :2].mean() < 0.6 * params["D0_intra"]
@@ -72,6 +75,7 @@ def test_gradient_driver_matches_oracle(substrate_file, oracle_lib):
     err = np.abs(res["signals"] - ref)
     print(f"signals vs oracle: max abs {err.max():.2e}")
     assert err.max() < 1e-3
+    assert err.max() < 1e-6  # equal trajectories: FP32 moments are all that differ
     # extra-axonal signal decays with b and stays above free diffusion's
     mag = np.abs(res["signals"]).reshape(2, 6)
     assert np.all(mag[1] < mag[0]) and np.all(mag[0] > np.exp(-1.0 * 2.0) - 0.01)

@@ -2,9 +2,11 @@
 Philox stream (the small-substrate configuration: 16 beaded axons in a 30 um
 periodic cube, dt = 20 us).
 
-Bars: per-spin trajectories within 1e-5 um over 1e3 steps for spins with at
-most 60 wall collisions (collision-rich spins separate chaotically, in the
-oracle itself too), per-direction signals within 1e-3 of the oracle.
+Bars: every spin's trajectory within 1e-5 um of the oracle's over 1e3 steps
+(the walker's FP64 steps repeat the oracle's operation for operation, so
+trajectories are bit-identical; wall collisions would amplify any rounding
+difference exponentially, see tests/test_cpu_core.py::test_oracle_is_chaotic),
+per-direction signals within 1e-3 of the oracle (measured: ~1e-8).
 """
 import numpy as np
 import pytest
@@ -17,7 +19,11 @@ from nozomi_amd.substrate import generate_beaded_axons, load_geometry
 
 pytestmark = pytest.mark.gpu
 
-SIGNAL_TOL = 1e-3    # absolute
+SIGNAL_TOL = 1e-3    # absolute, the per-direction signal bar
+TRAJ_TOL = 1e-5      # um, per spin over 1e3 steps
+# fraction of spins that must meet TRAJ_TOL: all but the rare three-way
+# near-tie of FP32 roots (two are re-solved in FP64)
+TRAJ_FRAC = 0.999
 
 
 @pytest.fixture(scope="module")
@@ -60,10 +66,8 @@ def test_seed_matches_oracle(c0_geometry, oracle_lib):
 
 @pytest.mark.parametrize("cell", [None, 4.0])
 def test_trajectory_parity_1000_steps(c0_geometry, oracle_lib, cell):
-    """Per-spin FP64 state against the oracle every 100 steps.  Wall
-    collisions amplify any rounding difference exponentially (see
-    tests/test_cpu_core.py::test_oracle_is_chaotic), so the 1e-5 um bar is
-    applied to spins with at most 60 scatter sub-steps and the rest reported.
+    """Per-spin FP64 state against the oracle every 100 steps: every spin
+    within 1e-5 um (bit-identical in practice), the scatter histories equal.
     Cells: the default edge, and 4 um (candidate lists of up to 66 entries,
     past the reach pass's 64-entry limit)."""
     g = c0_geometry
@@ -88,17 +92,19 @@ def test_trajectory_parity_1000_steps(c0_geometry, oracle_lib, cell):
                          np.abs(gd - (pos - pos0)).max(axis=0))
         few = nsub_o <= 60
         print(f"  steps={(c + 1) * chunk} within1e-5: all {np.mean(err < 1e-5):.4f} "
-              f"<=60 collisions {np.mean(err[few] < 1e-5):.4f} median {np.median(err):.2e} "
+              f"<=60 collisions {np.mean(err[few] < 1e-5):.4f} bit-identical "
+              f"{np.mean(err == 0):.4f} median {np.median(err):.2e} "
               f"history mismatches {int(np.sum(nsub_g != nsub_o))} "
               f"FP64 scans {int(nguard.sum())}", flush=True)
-        assert np.mean(err[few] < 1e-5) > 0.995
-    assert np.mean(err < 1e-5) > 0.85
+        assert np.mean(err < TRAJ_TOL) >= TRAJ_FRAC
+    assert np.mean(err == 0) >= 0.99
+    assert np.mean(nsub_g == nsub_o) >= TRAJ_FRAC
     # the float32 drop-in views agree to their own rounding
     gp32 = sim.spins_d.get().astype(np.float64)
     assert np.abs(_min_image(gp32 - gp, L)).max() < 2e-6
     gs = sim.sig_d.get().ravel()
-    few = nsub_o <= 60
-    assert np.abs(gs[few] / sig[few] - 1).max() < 1e-5
+    ok = err < TRAJ_TOL  # the T2 weight is an FP32 log sum: compare along equal paths
+    assert np.abs(gs[ok] / sig[ok] - 1).max() < 1e-5
 
 
 @pytest.mark.parametrize("dt,nsteps", [(0.3, 40), (1.0, 20)])
@@ -125,8 +131,7 @@ def test_large_steps_fp64_fallbacks(c0_geometry, oracle_lib, dt, nsteps):
     print(f"  within 1e-5: all {np.mean(err < 1e-5):.4f}, <=60 collisions "
           f"{np.mean(err[few] < 1e-5):.4f}; FP64 scans {int(nguard.sum())}", flush=True)
     assert nguard.sum() > 10000
-    assert np.mean(err[few] < 1e-5) > 0.995
-    assert np.mean(err < 1e-5) > 0.98
+    assert np.mean(err < TRAJ_TOL) >= TRAJ_FRAC
 
 
 def test_non_cubic_box_parity(oracle_lib):
@@ -158,8 +163,7 @@ def test_non_cubic_box_parity(oracle_lib):
     print(f"non-cubic: within 1e-5 all {np.mean(err < 1e-5):.4f}, <=60 collisions "
           f"{np.mean(err[few] < 1e-5):.4f} ({few.sum()} spins); collisions {int(nsub_o.sum())}", flush=True)
     assert nsub_o.sum() > 1000
-    assert np.mean(err[few] < 1e-5) > 0.995
-    assert np.mean(err < 1e-5) > 0.95
+    assert np.mean(err < TRAJ_TOL) >= TRAJ_FRAC
 
 
 def test_compartment_d_t2_parity(oracle_lib):
@@ -181,14 +185,12 @@ def test_compartment_d_t2_parity(oracle_lib):
     gp, gd = sim.walker.read_f64()
     err = np.maximum(np.abs(_min_image(gp - pos, L)).max(axis=0),
                      np.abs(gd - (pos - pos0)).max(axis=0))
-    few = nsub_o <= 60
+    ok = err < TRAJ_TOL
     gs = sim.sig_d.get().ravel().astype(np.float64)
-    print(f"compartments: within 1e-5 all {np.mean(err < 1e-5):.4f}, <=60 collisions "
-          f"{np.mean(err[few] < 1e-5):.4f}; T2 weight rel err {np.abs(gs[few] / sig[few] - 1).max():.2e}",
-          flush=True)
-    assert np.mean(err[few] < 1e-5) > 0.995
-    assert np.mean(err < 1e-5) > 0.95
-    assert np.abs(gs[few] / sig[few] - 1).max() < 1e-5
+    print(f"compartments: within 1e-5 all {np.mean(ok):.4f}, bit-identical {np.mean(err == 0):.4f}; "
+          f"T2 weight rel err {np.abs(gs[ok] / sig[ok] - 1).max():.2e}", flush=True)
+    assert np.mean(ok) >= TRAJ_FRAC
+    assert np.abs(gs[ok] / sig[ok] - 1).max() < 1e-5
     # both compartments' rates are in use
     assert sig.min() < np.exp(-T * dt / 80.0) * 1.001 and sig.max() > np.exp(-T * dt / 120.0) * 0.999
 
@@ -196,10 +198,6 @@ def test_compartment_d_t2_parity(oracle_lib):
 def test_phase_signal_parity(c0_geometry, oracle_lib):
     g = c0_geometry
     prot = PGSEShells(10.0, 30.0, 0.02, [1.0], icosahedron_6())
-    # collision-rich spins separate chaotically (test_cpu_core.py); their
-    # phases then differ but follow the same law, which leaves a statistical
-    # signal difference of order sqrt(n_diverged)/N: 65536 spins keep it
-    # well below the 1e-3 bar
     N = 65536
     sim = DwiSim3d(g, N, diffdir=prot.dirs)
     sim.setup(initstates_int=777)
@@ -222,8 +220,12 @@ def test_phase_signal_parity(c0_geometry, oracle_lib):
     err = np.abs(s_gpu - s_orc).max()
     print(f"signal |err| max {err:.3e}; phase err median {np.median(np.abs(ph_gpu - ph_orc)):.3e}")
     assert err < SIGNAL_TOL
+    # trajectories are the oracle's, so only the FP32 moment accumulation
+    # separates the phases
+    assert err < 1e-6
     assert np.abs(s_gpu - s_gpu_np).max() < 1e-5  # device FP64 reduction == host reduction
-    assert np.median(np.abs(ph_gpu - ph_orc)) < 1e-4
+    assert np.median(np.abs(ph_gpu - ph_orc)) < 1e-5
+    assert np.abs(ph_gpu - ph_orc).max() < 1e-3
 
 
 def test_free_diffusion_signal():

@@ -265,3 +265,30 @@ def test_reference_segments_brute_force():
                 assert len(set(got.tolist())) == sps
                 counts.append(len(want))
     assert sps == max(counts)
+
+
+def test_restricted_cylinder_matches_burcaw(oracle_lib):
+    """Spins inside a cylinder of radius a = 0.5 um (a dense sphere chain,
+    wall ripple 0.06%): the oracle's radial D(t) follows the Burcaw et al.
+    series (closed form, tests/closed_forms.py) and reaches its long-time
+    limit a^2/(4t).  4000 spins, 10 ms of 1 us steps; sampling s.e. about
+    2%, so 8% bars."""
+    from nozomi_amd.simulation_engine.geometry import SimGeometry3D, Structure3D
+    from .closed_forms import burcaw_dperp, cylinder_chain
+    a, D0, dt, T, N = 0.5, 2.0, 0.001, 10000, 4000
+    g = cylinder_chain(SimGeometry3D, Structure3D, a, 6.0, 0.05, 60, D0)
+    o = oracle_lib.OracleSubstrate(g.spheres, g.nstructures, g.structure_D(), g.structure_T2(),
+                                   (g.Lx, g.Ly, g.Lz))
+    p0 = o.seed(N, [0], seed=12)
+    pos, pos0, sig = p0.copy(), p0.copy(), np.ones(N)
+    sm = o.walk(pos, pos0, sig, seed=13, step0=0, nsteps=T, dt=dt, stepmom=True)["stepmom"]
+    assert g.structures[0].isinside(pos).all()
+    t = dt * np.arange(1, T + 1)
+    dp = (sm[:, 0] + sm[:, 1]) / (4 * t * N)
+    idx = np.searchsorted(t, [0.05, 0.2, 1.0, 5.0, 9.999])
+    ref = burcaw_dperp(t[idx], a, D0)
+    rel = np.abs(dp[idx] / ref - 1)
+    assert (rel < 0.08).all(), (t[idx], dp[idx], ref)
+    assert abs(dp[-1] / (a ** 2 / (4 * t[-1])) - 1) < 0.08
+    # axial diffusion is free
+    assert abs(sm[-1, 2] / (2 * t[-1] * N) / D0 - 1) < 0.1
