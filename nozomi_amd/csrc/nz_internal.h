@@ -17,6 +17,11 @@ constexpr float kLarge = 1e6f;
 constexpr int kMaxIter = 100;
 // Longest candidate list of a cell (the FP32 scan keys a list position in 8 bits).
 constexpr int kMaxListLen = 255;
+// Rejection-seeding attempts per spin before it counts as unplaceable: a
+// target filling 0.2% of the box (a 0.5 um cylinder in a 20 um box) needs
+// ~500 on average, and a spin's attempts are keyed (k, 0, spin), so a larger
+// bound changes no spin that an earlier one placed.
+constexpr int32_t kSeedAttempts = 1 << 20;
 // Zero records after the last candidate list (the scans load up to 15
 // records past a list's end).
 constexpr int kCandPad = 16;

@@ -1337,7 +1337,7 @@ int nz_batch_add(nz_batch_t bb, nz_substrate_t sub, int64_t n, uint64_t seed, in
   const float* p0 = pos;
   if (!p0) {
     hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, b->stream, b->nfailed_dev, (int64_t)1, 0);
-    if (int rc = nz_seed(sub, n, targets_host, ntargets, seed ^ 0x5EED5EED5EEDull, 4096,
+    if (int rc = nz_seed(sub, n, targets_host, ntargets, seed ^ 0x5EED5EED5EEDull, kSeedAttempts,
                          b->pos_tmp, b->nfailed_dev, b->stream))
       return rc;
     int32_t nf = 0;

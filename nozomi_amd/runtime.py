@@ -14,6 +14,11 @@ from ._lib import NzSubstrateInfo, check, lib
 
 _c_vp = ctypes.c_void_p
 
+# rejection-seeding attempts per spin (as kSeedAttempts in the library: a
+# target filling 0.2% of the box needs ~500 on average; attempts are keyed
+# per spin, so the bound changes no spin that fewer attempts placed)
+SEED_ATTEMPTS = 1 << 20
+
 
 def _require_gpu(device):
     if not torch.cuda.is_available():
@@ -164,7 +169,7 @@ class DeviceSubstrate:
         except Exception:
             pass
 
-    def seed(self, n, targets=None, seed=0, max_attempts=4096, out=None):
+    def seed(self, n, targets=None, seed=0, max_attempts=SEED_ATTEMPTS, out=None):
         """Rejection-seed n spins into the target structures; returns [3, n]."""
         pos = out if out is not None else torch.empty((3, n), dtype=torch.float32, device=self.device)
         if pos.dtype != torch.float32 or pos.numel() < 3 * n:
@@ -238,7 +243,7 @@ class Walker:
                   "nz_walker_set_positions")
         self._keep = pos
 
-    def seed(self, targets=None, seed=0, max_attempts=4096):
+    def seed(self, targets=None, seed=0, max_attempts=SEED_ATTEMPTS):
         failed = torch.zeros(1, dtype=torch.int32, device=self.device)
         t = None if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
         with torch.cuda.device(self.device):
