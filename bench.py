@@ -47,12 +47,15 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--forms", type=int, default=1,
                     help="also measure the batched and served forms (scaled; 0 = skip)")
-    ap.add_argument("--forms-steps", type=float, default=0.1,
-                    help="step-count scale of the batched/served workloads")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--forms-steps", type=float, default=1.0,
+                    help="step-count scale of the batched/served workloads (1.0 = as stated)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="arrival window of the served stream (the stated stream is 120 s)")
     ap.add_argument("--precision-spins", type=int, default=4_000_000)
-    ap.add_argument("--precision-steps", type=int, default=2000,
+    ap.add_argument("--precision-steps", type=int, default=10000,
                     help="steps of the precision-form measurement (1e4 = the full workload)")
+    ap.add_argument("--precision-fp32-steps", type=int, default=1024,
+                    help="steps of the FP32 vector-ALU contraction run (signal comparison)")
     ap.add_argument("--precision-waveforms", type=int, default=1024)
     ap.add_argument("--only", choices=("precision", "batching", "serving"), default=None,
                     help="measure only this form and print its JSON object")
@@ -175,7 +178,7 @@ def main():
     launch_units = float(N) * steps_per_launch
     achieved_tf = launch_units * flop_per / (launch_avg_ms * 1e-3) / 1e12
     hbm_gbs = launch_units * 24.0 / (launch_avg_ms * 1e-3) / 1e9
-    traffic = measured_traffic(N, steps_per_launch, info)
+    traffic, traffic_note = measured_traffic(N, steps_per_launch, info)
 
     out = {
         "metric": "spin-steps/s at 1e6 spins",
@@ -188,7 +191,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp64 state / fp32 scan",
         "data": "synthetic (seeded beaded-axon substrate, uniform spins)",
         "config": {
             "workload": "plain walker FP32 + PGSE phase: 200 axons, 60 um box, 1e6 spins, "
@@ -197,6 +200,9 @@ def main():
             "spheres": int(info["nspheres"]), "structures": int(info["nstructures"]),
             "cells": list(info["ncell"]), "cell_um": info["cell"][0], "margin_um": info["margin"],
             "mean_candidates": cbar, "scatter_per_step": sbar, "sort_interval": args.sort,
+            "precision": "FP64 position, jump and deciding roots in the reference's frame and "
+                         "operation order (trajectories bit-identical to the FP64 oracle); FP32 "
+                         "candidate scan; FP32 phase moment; FP64 signal sums",
         },
         "walk_ms_per_pass": kern_ms,
         "walk_spin_steps_per_s": float(N) * T / (kern_ms * 1e-3),
@@ -205,7 +211,8 @@ def main():
         "roofline": {
             "bound": "valu", "achieved": achieved_tf, "peak": FP32_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved_tf / FP32_VECTOR_PEAK_TFLOPS,
-            "traffic": traffic, "kernel": "walk_kernel", "flop_per_spin_step": flop_per,
+            "traffic": traffic, "traffic_source": traffic_note, "kernel": "walk_kernel",
+            "flop_per_spin_step": flop_per,
             "spin_steps_per_launch": launch_units,
         },
         "roofline_hbm": {
@@ -228,20 +235,22 @@ def main():
 
 
 def precision_form(args, geom, sub):
-    """W waveform shapes contracted with K = 128-step blocks of displacements:
-    split hi/lo BF16 MFMA against the FP32 vector-ALU contraction."""
+    """W waveform shapes contracted with K-step blocks of displacements (split
+    hi/lo BF16 MFMA, FP32 accumulation): K = 64 (the stated configuration) and
+    K = 128 over the whole walk, the contraction of block b overlapping the
+    walk of block b + 1; each contraction also timed alone on the first
+    blocks.  The FP32 vector-ALU contraction runs a shorter walk for the
+    signal comparison."""
     import torch
     from nozomi_amd.precision import PrecisionRun, precision_waveforms
-    N, T, W, K, dt = args.precision_spins, args.precision_steps, args.precision_waveforms, 128, 0.01
+    N, T, W, dt = args.precision_spins, args.precision_steps, args.precision_waveforms, 0.01
     g, u, kinds = precision_waveforms(W, T, dt, seed=args.seed)
     x0 = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=args.seed + 3)
-    res = {"spins": N, "steps": T, "waveforms": W, "K": K, "dt_ms": dt,
-           "workload": "200-axon substrate, 4e6 spins, 1024 PGSE/OGSE waveforms (25-150 Hz, "
-                       "|G| <= 300 mT/m, seeded directions), steps as given"}
-    sig = {}
-    # the contraction kernels timed alone (no overlap), then the whole form
-    # with the contraction of block b overlapping the walk of block b + 1
-    for mode, overlap in (("mfma", False), ("fp32", False), ("mfma", True)):
+    res = {"spins": N, "steps": T, "waveforms": W, "dt_ms": dt,
+           "workload": "200-axon substrate, 4e6 spins, 1e4 steps, 1024 PGSE/OGSE waveforms "
+                       "(25-150 Hz, |G| <= 300 mT/m, seeded directions)"}
+
+    def timed(K, mode, steps, overlap, nblocks_timed=None):
         pr = PrecisionRun(sub, x0, args.seed, g, u, dt, K=K, mode=mode, overlap=overlap)
         pr.reset()
         pr.block(0)  # warm-up block (kernels, sort)
@@ -250,34 +259,58 @@ def precision_form(args, geom, sub):
         if not overlap:
             pr.gemm_events = []
         t0 = time.perf_counter()
-        for b0 in range(0, T, K):
+        for b0 in range(0, steps, K):
             pr.block(b0)
         s = pr.signals()
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        if overlap:
-            res[mode]["overlapped_seconds"] = el
-            res[mode]["overlapped_spin_steps_per_s"] = N * T / el
-            sig[mode + "_overlap"] = s
-        else:
-            sig[mode] = s
-            useful = 6.0 * W * N * K  # FLOP per full block
-            full = [a.elapsed_time(b) for (a, b), b0 in zip(pr.gemm_events, range(0, T, K))
-                    if b0 + K <= T]
-            avg = float(np.mean(full))
-            res[mode] = {"seconds": el, "spin_steps_per_s": N * T / el,
-                         "contraction_ms_per_block": avg,
-                         "contraction_useful_tflops": useful / (avg * 1e-3) / 1e12}
+        ms = None
+        if not overlap:
+            full = [a.elapsed_time(b) for (a, b), b0 in zip(pr.gemm_events, range(0, steps, K))
+                    if b0 + K <= steps]
+            ms = float(np.mean(full[:nblocks_timed] if nblocks_timed else full))
         pr.close()
         del pr
         torch.cuda.empty_cache()
-    peak = 2500.0  # dense BF16 MFMA TFLOP/s
-    m = res["mfma"]
-    m["contraction_executed_tflops"] = 3 * m["contraction_useful_tflops"]
-    m["roofline"] = {"bound": "mfma", "achieved": m["contraction_executed_tflops"], "peak": peak,
-                     "unit": "TFLOP/s", "frac": m["contraction_executed_tflops"] / peak}
-    res["max_signal_diff_mfma_vs_fp32"] = float(np.abs(sig["mfma"] - sig["fp32"]).max())
-    res["max_signal_diff_overlap_vs_serial"] = float(np.abs(sig["mfma_overlap"] - sig["mfma"]).max())
+        return el, s, ms
+
+    peak_mfma, peak_hbm = 2500.0, HBM_PEAK_GBS
+    prog = _progress("precision")
+    for K in (64, 128):
+        # the contraction alone (one stream) on the first 1280 steps
+        nt = min(T, 1280)
+        _, _, ms = timed(K, "mfma", nt, overlap=False)
+        prog(K, 128)
+        el, sig, _ = timed(K, "mfma", T, overlap=True)
+        prog(K, 128)
+        useful = 6.0 * W * N * K            # FLOP of one block (3 MAC per waveform per step)
+        executed = 3.0 * useful             # hi.hi + hi.lo + lo.hi
+        # HBM floor of a block: the record read (3K FP32 per spin) and the phase
+        # read-modify-write (2 x W FP32 per spin)
+        hbm_bytes = N * (12.0 * K + 8.0 * W)
+        res[f"K{K}"] = {
+            "K": K, "seconds": el, "spin_steps_per_s": N * T / el, "signals_per_s": W / el,
+            "contraction_ms_per_block": ms,
+            "contraction_useful_tflops": useful / (ms * 1e-3) / 1e12,
+            "roofline": {"bound": "mfma", "achieved": executed / (ms * 1e-3) / 1e12,
+                         "peak": peak_mfma, "unit": "TFLOP/s",
+                         "frac": executed / (ms * 1e-3) / 1e12 / peak_mfma},
+            "roofline_hbm": {"bound": "hbm", "bytes_per_block": hbm_bytes,
+                             "bytes_per_spin_step": hbm_bytes / (N * K),
+                             "achieved": hbm_bytes / (ms * 1e-3) / 1e9, "peak": peak_hbm,
+                             "unit": "GB/s", "frac": hbm_bytes / (ms * 1e-3) / 1e9 / peak_hbm,
+                             "floor_ms_per_block": hbm_bytes / (peak_hbm * 1e9) * 1e3},
+        }
+        res[f"K{K}"]["_signals"] = sig
+    # FP32 vector-ALU contraction against the MFMA one on the same shorter walk
+    nt = min(T, args.precision_fp32_steps)
+    _, s_f32, ms32 = timed(128, "fp32", nt, overlap=False)
+    _, s_mfma, _ = timed(128, "mfma", nt, overlap=False)
+    res["fp32_contraction"] = {"steps": nt, "K": 128, "contraction_ms_per_block": ms32,
+                               "useful_tflops": 6.0 * W * N * 128 / (ms32 * 1e-3) / 1e12}
+    res["max_signal_diff_mfma_vs_fp32"] = float(np.abs(s_mfma - s_f32).max())
+    res["max_signal_diff_K64_vs_K128"] = float(np.abs(res["K64"].pop("_signals") -
+                                                      res["K128"].pop("_signals")).max())
     return res
 
 
@@ -360,28 +393,77 @@ def forms(args):
     return {"batching": batching_form(args), "serving": serving_form(args)}
 
 
+def full_occupancy_rate(reqs, cache, seed, nsample=6, n=393216, steps=288):
+    """Spin-steps/s of a walk that fills the GPU (n spins: two rounds of the
+    resident wave slots) on a sample of the mix's substrates, with their
+    tuned sort interval: what any schedule of the mix can reach at most per
+    spin-step.  Returns (max, mean) over the sample."""
+    import torch
+    from nozomi_amd import runtime as rt
+    rng = np.random.default_rng(seed)
+    keys = list(dict.fromkeys(r.key for r in reqs))
+    rates = []
+    for i in rng.choice(len(keys), size=min(nsample, len(keys)), replace=False):
+        r = next(q for q in reqs if q.key == keys[i])
+        sub, geom = cache.acquire(r.key, r.make_geometry)
+        w = rt.Walker(sub, n, sort_interval=cache.params(r.key)["k"])
+        w.seed(targets=np.arange(geom.nstructures + 1), seed=seed + int(i))
+        w.walk(seed, 96, r.dt, t2=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        w.walk(seed, steps, r.dt, t2=False)
+        torch.cuda.synchronize()
+        rates.append(n * steps / (time.perf_counter() - t0))
+        w.close()
+        cache.release(r.key)
+    return float(max(rates)), float(np.mean(rates))
+
+
 def batching_form(args):
     import torch
     from nozomi_amd.serve import SubstrateCache, run_batched, run_serial
     from nozomi_amd.workloads import batching_mix, pregenerate, total_spin_steps
     dev = torch.device("cuda:0")
     # batching: 256 independent simulations of uneven size in one pool; their
-    # substrates are generated and uploaded before timing (resident in HBM)
+    # substrates are generated, tuned and uploaded before timing (resident)
     reqs = batching_mix(nsims=256, seed=args.seed, scale_steps=args.forms_steps)
     pregenerate(reqs)
-    cache = SubstrateCache(dev, 1 << 40)
+    cache = SubstrateCache(dev, 1 << 40, tune=True)
     cache.prefetch(reqs)
+    from nozomi_amd.autotune import family_key
+    fam = {}
+    for v in cache.items.values():
+        fam.setdefault(str(family_key(v[1])), {"cell": v[4]["cell"], "margin": v[4]["margin"],
+                                               "k": v[4]["k"], "substrates": 0})
+        fam[str(family_key(v[1]))]["substrates"] += 1
     cap = sum(-(-r.n // 256) * 256 for r in reqs)
     run_serial(reqs[:4], cache=cache)  # warms the kernels
     got, t_b = run_batched(reqs, capacity=cap, k=96, cache=cache, progress=_progress("batched"))
     alone, t_s = run_serial(reqs, cache=cache, progress=_progress("serial"))
     diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
     ss = total_spin_steps(reqs)
+    # what any schedule could gain: each simulation at the full-occupancy rate
+    rmax, rmean = full_occupancy_rate(reqs, cache, args.seed)
+    dur = np.array([alone[r.rid].latency for r in reqs])
+    work = np.array([float(r.spin_steps) for r in reqs])
+    lanes = 256 * 4 * 3 * 64  # resident lanes of the walk kernel (3 waves per SIMD)
+    under = np.array([r.n < lanes for r in reqs])
     return {
         "sims": len(reqs), "spin_steps": ss, "step_scale": args.forms_steps,
         "batched_sims_per_s": len(reqs) / t_b, "serial_sims_per_s": len(reqs) / t_s,
         "speedup_vs_serial": t_s / t_b, "batched_spin_steps_per_s": ss / t_b,
         "serial_spin_steps_per_s": ss / t_s, "max_signal_diff_vs_alone": diff,
+        "autotuned": fam,
+        "amdahl": {
+            "full_occupancy_spin_steps_per_s_max": rmax,
+            "full_occupancy_spin_steps_per_s_mean": rmean,
+            "speedup_bound": float(dur.sum() / (work.sum() / rmax)),
+            "serial_seconds": float(dur.sum()),
+            "serial_share_of_underfilled_sims": float(dur[under].sum() / dur.sum()),
+            "underfilled_sims": int(under.sum()), "underfilled_below_spins": lanes,
+            "note": "bound = serial time / (spin-steps / best full-occupancy rate): no "
+                    "schedule walks a spin-step faster than a GPU-filling plain walk",
+        },
     }
 
 
@@ -394,11 +476,11 @@ def serving_form(args):
                             scale_steps=args.forms_steps)
     pregenerate(reqs)
     cap = 16 * 1024 * 1024
-    srv = Server(cap, hbm_budget=64 << 30, max_sims=256, k=64)
+    srv = Server(cap, hbm_budget=64 << 30, max_sims=256, k=64, tune=True)
     got, makespan = serve_trace(srv, reqs, arr, progress=_progress("served"))
     built, reused = srv.subs.built, srv.subs.reused
     srv.close()
-    alone, t_s = run_serial(reqs, progress=_progress("serial"))
+    alone, t_s = run_serial(reqs, cache=srv.subs, progress=_progress("serial"))
     diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
     ss = total_spin_steps(reqs)
     lat_serial = fifo_latencies(arr, [alone[r.rid].latency for r in reqs])
@@ -415,21 +497,37 @@ def serving_form(args):
     }
 
 
+WALK_SOURCES = ("nz_walk.hip", "nz_walk_core.h", "nz_internal.h", "nz_rng.h")
+
+
+def walk_source_hash():
+    """sha256 (first 16 hex digits) of the walk kernel's sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in WALK_SOURCES:
+        with open(os.path.join(ROOT, "nozomi_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(n, steps_per_launch, info):
     """HBM bytes per walk-kernel launch from the committed counter summary
     (profiles/walk_kernel_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch,
     scaled by the copy-kernel calibration), when it was measured on this
-    workload shape; else None."""
+    workload shape and on the walk kernel built from the current sources;
+    else None.  Returns (bytes or None, why)."""
     path = os.path.join(ROOT, "profiles", "walk_kernel_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, "no counter summary"
     if (t.get("spins") != n or t.get("steps_per_launch") != steps_per_launch
             or t.get("cells") != list(info["ncell"])):
-        return None
-    return t.get("bytes_per_launch")
+        return None, "counter summary measured on another workload shape"
+    if t.get("sources_sha16") != walk_source_hash():
+        return None, "counter summary measured on other walk-kernel sources"
+    return t.get("bytes_per_launch"), t.get("source")
 
 
 def measure_sbar(geom, sub, prot, dev, seed, n=65536, steps=200):
@@ -492,9 +590,7 @@ def oracle_check(gpu, orc, prot, L, T):
 
     sg, so = sig(mom_g), sig(mom_o)
     ds = np.abs(sg - so)
-    # standard error of one side's sample mean of exp(i phase): once the
-    # trajectories have decorrelated, the difference of the two sides is
-    # sampling noise of about sqrt(2) times this
+    # standard error of one side's sample mean of exp(i phase), for scale
     se = np.sqrt(np.maximum(1.0 - np.abs(so) ** 2, 0.0) / err.size)
     return {
         "spins": int(err.size), "steps": int(T),
@@ -502,13 +598,13 @@ def oracle_check(gpu, orc, prot, L, T):
                                 "p90": float(np.percentile(err, 90)),
                                 "p99": float(np.percentile(err, 99)), "max": float(err.max())},
         "fraction_within_1e-5_um": float(np.mean(err <= 1e-5)),
+        "fraction_bit_identical": float(np.mean(err == 0.0)),
         "signal_max_abs_diff": float(ds.max()), "signal_mean_abs_diff": float(ds.mean()),
+        "signal_bar_1e-3_met": bool(ds.max() < 1e-3),
         "signal_sampling_se_mean": float(se.mean()),
-        "signal_max_diff_over_se": float((ds / np.maximum(se, 1e-30)).max()),
-        "note": "wall collisions are chaotic: over 5000 steps (about 670 collisions per spin) "
-                "the trajectories of two differently rounded walks decorrelate, so the signal "
-                "difference of a finite sample is sampling noise; oracle_check_1000_steps is "
-                "the trajectory check",
+        "note": "GPU (FP64 state, FP32 scan) against the FP64 oracle on the same spins, start "
+                "positions and Philox stream, all 5000 steps; the signals differ only by the "
+                "GPU's FP32 phase-moment accumulation",
     }
 
 
@@ -543,6 +639,7 @@ def trajectory_check(geom, sub, prot, start, args, n=20000, T=1000):
     return {
         "spins": int(n), "steps": int(T),
         "fraction_within_1e-5_um": float(np.mean(err <= 1e-5)),
+        "fraction_bit_identical": float(np.mean(err == 0.0)),
         "spins_with_at_most_60_scatter_substeps": int(few.sum()),
         "fraction_within_1e-5_um_at_most_60": float(np.mean(err[few] <= 1e-5)),
         "median_error_um_at_most_60": float(np.median(err[few])),

@@ -6,6 +6,11 @@ re-bins), the list margin (list length against safety-box fallbacks) and the
 steps per launch, which is also the spatial re-sort interval (sort cost
 against wavefront coherence).  ``tune`` times a short walk of a subsample for
 every combination and returns the fastest; results are cached per key.
+
+``tuned`` is what the batched and served forms call for every substrate: it
+tunes once per substrate family (packing density and sphere size, which set
+the candidate counts and collision rates the choices trade against) and
+reuses the choice for every substrate of the family.
 """
 import itertools
 import time
@@ -45,3 +50,20 @@ def tune(geom, key=None, n=65536, steps=128, dt=0.01, cells=(1.5, 1.75, 2.0, 2.2
     if key is not None:
         _CACHE[key] = out
     return out
+
+
+def family_key(geom):
+    """Substrates whose sphere density (spheres per um^3, in steps of 2^0.25)
+    and mean sphere radius (in steps of 0.1 um) agree share their tuning."""
+    sp = np.asarray(geom.spheres)
+    if sp.shape[1] == 0:
+        return ("empty",)
+    vol = float(geom.Lx) * float(geom.Ly) * float(geom.Lz)
+    dens = sp.shape[1] / vol
+    return (int(round(4 * np.log2(dens))), int(round(10 * float(sp[3].mean()))))
+
+
+def tuned(geom, device=None, **kw):
+    """The tuned (cell, margin, k) of the geometry's family (tunes on the first
+    geometry of a family, then reuses it)."""
+    return tune(geom, key=("family",) + family_key(geom), device=device, **kw)

@@ -122,14 +122,23 @@ __global__ void __launch_bounds__(256) signal_kernel(int64_t n, int64_t ld, cons
   }
 }
 
-// out[e] = sum over the nb blocks of partial[b][e], in block order (2W threads).
+// out[e] = sum over the nb blocks of partial[b][e], one workgroup per e:
+// thread t adds blocks t, t + 256, ... in order, then a fixed tree over the
+// 256 thread sums (the same order on every call).
 __global__ void __launch_bounds__(256) signal_sum_kernel(const double* partial, int64_t nb,
                                                          int32_t W, double* out) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 2 * (int64_t)W) return;
+  __shared__ double red[256];
+  const int64_t e = blockIdx.x;
   double s = 0.0;
-  for (int64_t b = 0; b < nb; ++b) s += partial[b * 2 * W + e];
-  out[e] = s;
+  for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) s += partial[b * 2 * W + e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+#pragma unroll
+  for (int o = 128; o >= 1; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[e] = red[0];
 }
 
 int64_t signal_partial_len(int64_t n, int32_t W) {
@@ -147,7 +156,7 @@ int signal_reduce(int64_t n, int64_t ld, const int32_t* valid, int32_t W, const 
   signal_partition(n, &nb, &tpb);
   hipLaunchKernelGGL(signal_kernel, dim3((unsigned)nb), dim3(256), 0, stream, n, ld, valid, W, mom,
                      dirs, scale, tpb, partial);
-  hipLaunchKernelGGL(signal_sum_kernel, dim3((unsigned)((2 * W + 255) / 256)), dim3(256), 0, stream,
+  hipLaunchKernelGGL(signal_sum_kernel, dim3((unsigned)(2 * W)), dim3(256), 0, stream,
                      (const double*)partial, nb, W, out);
   NZ_HIP(hipGetLastError());
   return 0;

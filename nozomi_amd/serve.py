@@ -80,13 +80,17 @@ class Result:
 class SubstrateCache:
     """Device substrates by key, built on first use, kept while referenced or
     while the byte budget allows (least recently used evicted first).
-    ``prefetch`` builds them ahead (host geometry + device upload)."""
+    ``prefetch`` builds them ahead (host geometry + device upload).  With
+    tune=True the cell edge, list margin and steps per launch of a substrate
+    come from the autotuner (per substrate family); ``params(key)`` returns
+    them."""
 
-    def __init__(self, device, budget_bytes, cell_size=0.0, margin=0.0):
+    def __init__(self, device, budget_bytes, cell_size=0.0, margin=0.0, tune=False):
         self.device = device
         self.budget = budget_bytes
         self.cell_size, self.margin = cell_size, margin
-        self.items = collections.OrderedDict()   # key -> [sub, geom, bytes, refs]
+        self.tune = tune
+        self.items = collections.OrderedDict()   # key -> [sub, geom, bytes, refs, params]
         self.bytes = 0
         self.built = 0
         self.reused = 0
@@ -98,13 +102,22 @@ class SubstrateCache:
             self.reused += 1
             return self.items[key][0], self.items[key][1]
         geom = make_geometry()
-        sub = geom.device_substrate(cell_size=self.cell_size, margin=self.margin, device=self.device)
+        params = {"cell": self.cell_size, "margin": self.margin, "k": None}
+        if self.tune:
+            from .autotune import tuned
+            params = tuned(geom, device=self.device)
+        sub = geom.device_substrate(cell_size=params["cell"], margin=params["margin"],
+                                    device=self.device)
         nbytes = int(sub.info()["device_bytes"])
-        self.items[key] = [sub, geom, nbytes, 1]
+        self.items[key] = [sub, geom, nbytes, 1, params]
         self.bytes += nbytes
         self.built += 1
         self._evict()
         return sub, geom
+
+    def params(self, key):
+        """(cell, margin, k) the substrate was built with (k None: untuned)."""
+        return self.items[key][4]
 
     def prefetch(self, requests):
         for r in requests:
@@ -132,17 +145,18 @@ class Server:
     capacity: spin slots of the pool; hbm_budget: bytes the server may hold
     for substrates and per-request protocol buffers beside the pool; k: steps
     per launch; launches: launches per scheduling round (2 replays the
-    captured pair)."""
+    captured pair); tune: take cell edge, list margin and steps per launch
+    from the autotuner (per substrate family) instead of the defaults and k."""
 
     def __init__(self, capacity, hbm_budget=8 << 30, max_sims=256, k=64, launches=2,
-                 device=None, graph=True, cell_size=0.0, margin=0.0, cache=None):
+                 device=None, graph=True, cell_size=0.0, margin=0.0, cache=None, tune=False):
         self.device = rt._require_gpu(device)
         self.batch = Batch(capacity, max_sims=max_sims, device=self.device, graph=graph)
         self.k = int(k)
         self.launches = int(launches)
         self.hbm_budget = int(hbm_budget)
         self.subs = cache if cache is not None else SubstrateCache(
-            self.device, self.hbm_budget // 2, cell_size, margin)
+            self.device, self.hbm_budget // 2, cell_size, margin, tune=tune)
         self.queue = collections.deque()
         self.running = {}      # sim -> (request, admitted, tensors)
         self.results = {}
@@ -178,9 +192,10 @@ class Server:
             dirs = torch.from_numpy(np.ascontiguousarray(req.dirs, np.float32)).to(self.device)
             sc = torch.from_numpy(np.ascontiguousarray(req.scales, np.float32)).to(self.device)
             targets = req.targets if req.targets is not None else np.arange(geom.nstructures + 1)
+            k = self.subs.params(req.key)["k"] or self.k
             try:
                 sim = self.batch.add(sub, req.n, req.seed, req.nsteps, req.dt, g=g, garea=ga,
-                                     k=self.k, targets=targets)
+                                     k=k, targets=targets)
             except BatchFull:
                 self.subs.release(req.key)
                 return admitted
@@ -272,11 +287,11 @@ def serve_trace(server, requests, arrivals, poll=0.0, progress=None):
 
 
 def run_batched(requests, capacity, k=96, launches=2, device=None, graph=True, max_sims=256,
-                cache=None, progress=None):
+                cache=None, progress=None, tune=False):
     """Every request submitted at once, run to completion by one Server.
     Returns (results, seconds)."""
     srv = Server(capacity, hbm_budget=1 << 40, max_sims=max_sims, k=k, launches=launches,
-                 device=device, graph=graph, cache=cache)
+                 device=device, graph=graph, cache=cache, tune=tune)
     try:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -306,7 +321,7 @@ def run_serial(requests, device=None, cache=None, progress=None):
         report(len(results))
         t_start = time.perf_counter()
         sub, geom = cache.acquire(r.key, r.make_geometry)
-        w = rt.Walker(sub, r.n)
+        w = rt.Walker(sub, r.n, sort_interval=cache.params(r.key)["k"])
         targets = r.targets if r.targets is not None else np.arange(geom.nstructures + 1)
         w.seed(targets=targets, seed=r.seed ^ 0x5EED5EED5EED)
         g = torch.from_numpy(np.ascontiguousarray(r.g, np.float32)).to(device)

@@ -117,3 +117,46 @@ def test_overlapped_blocks_equal_serial():
         pr.close()
     assert np.array_equal(out[False][0], out[True][0])
     assert np.array_equal(out[False][1], out[True][1])
+
+
+def test_precision_form_full_size_two_blocks():
+    """The precision configuration's size: 4e6 spins on the 200-axon
+    substrate, 1024 waveforms, two K = 64 blocks.  The phases of a 4096-spin
+    subsample equal an FP64 contraction of the same recorded displacements to
+    the split-BF16 bound; every phase is finite."""
+    M, L = generate_beaded_axons(200, 60.0, seed=20261203)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    sub = geom.device_substrate(device=DEV)
+    N, T, dt, W, K, seed = 4_000_000, 128, 0.01, 1024, 64, 77
+    g, u, _ = precision_waveforms(W, T, dt, seed=20261203)
+    x0 = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=5)
+    pick = torch.arange(0, N, N // 4096, device=DEV)[:4096]  # spin ids of the subsample
+    blocks = []
+
+    def hook(t0, k, rec, slot_spin):
+        slot_of = torch.empty(N, dtype=torch.int64, device=DEV)
+        slot_of[slot_spin.long()] = torch.arange(N, device=DEV)
+        cols = slot_of[pick]
+        blocks.append((t0, k, rec[:3 * k].index_select(1, cols).cpu().numpy().astype(np.float64)))
+
+    print(f"precision form: {N} spins x {T} steps, {W} waveforms, K = {K}", flush=True)
+    pr = PrecisionRun(sub, x0, seed, g, u, dt, K=K, mode="mfma")
+    pr.run(record_hook=hook)
+    pr.finish()
+    ph = pr.phase.index_select(0, pick).cpu().numpy().astype(np.float64)
+    finite = bool(torch.isfinite(pr.phase).all().item())
+    pr.close()
+    assert finite and len(blocks) == 2
+    g64, u64 = g.astype(np.float64), u.astype(np.float64)
+    x0h = x0.index_select(1, pick).cpu().numpy().astype(np.float64)
+    ref = (x0h.T @ u64) * g64.sum(1)[None, :]
+    scale = (np.abs(x0h).T @ np.abs(u64)) * np.abs(g64).sum(1)[None, :]
+    for t0, k, d in blocks:
+        d = d.reshape(k, 3, -1)
+        ref += np.einsum("tas,wt,aw->sw", d, g64[:, t0:t0 + k], u64)
+        scale += np.einsum("tas,wt,aw->sw", np.abs(d), np.abs(g64[:, t0:t0 + k]), np.abs(u64))
+    err = np.abs(ph - ref)
+    print(f"  max |phase - FP64| = {err.max():.2e} rad; relative to sum |terms| "
+          f"{(err / (scale + 1e-30)).max():.2e}", flush=True)
+    assert (err / (scale + 1e-30)).max() < 6e-5
+    assert err.max() < 2e-3
