@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define NZ_ABI_VERSION 2
+#define NZ_ABI_VERSION 3
 
 typedef struct nz_substrate* nz_substrate_t;
 
@@ -228,17 +228,20 @@ int nz_batch_signals(nz_batch_t b, int32_t sim, int32_t W, const float* dirs, co
  * block's unwrapped displacements (nz_walk_record) and Bt[w][3(t-t0)+a] =
  * g_w(t) u_w,a (nz_waveform_block). */
 /* Walk nsteps (<= 4096) with every spin stopping at the block end, recording
- * FP32 displacements from the start: rec [3*nsteps][n] (row 3t+a, slot
- * order); slot_spin [n] receives the slot -> spin map of the record. */
+ * FP32 displacements from the start: rec [n][ld] slot-major, entry 3t + a of
+ * slot s the displacement along axis a after step t of the block (ld >= 3
+ * nsteps floats per slot); slot_spin [n] receives the slot -> spin map of the
+ * record. */
 int nz_walk_record(nz_walker_t w, uint64_t seed, int32_t nsteps, double dt, float* rec,
-                   int32_t* slot_spin, void* stream);
+                   int64_t ld, int32_t* slot_spin, void* stream);
 /* Waveform block for steps t0..t0+K-1: g [W][T] per-step coefficients,
  * u [3][W] directions; Bt as BF16 hi/lo [W][3K] (uint16 bits) and, if
  * bt_f32 is not NULL, FP32. */
 int nz_waveform_block(const float* g, int64_t T, const float* u, int32_t W, int64_t t0, int32_t K,
                       uint16_t* bt_hi, uint16_t* bt_lo, float* bt_f32, void* stream);
-/* phase [n_spins][W] += rec^T . Bt over the block: mode 0 = split hi/lo BF16
- * MFMA (K a multiple of 32, at most 128), mode 1 = FP32 vector ALUs. */
+/* phase [n_spins][W] += rec . Bt^T over the block, rec [nslots][ld] as
+ * nz_walk_record writes it (ld >= 3K): mode 0 = split hi/lo BF16 MFMA (K a
+ * multiple of 32, at most 128), mode 1 = FP32 vector ALUs. */
 int nz_phase_gemm(const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin,
                   int32_t K, const uint16_t* bt_hi, const uint16_t* bt_lo, const float* bt_f32,
                   int32_t W, int32_t mode, float* phase, void* stream);

@@ -64,14 +64,14 @@ constexpr int kTileW = 64;   // waveforms per workgroup tile
 constexpr int kChunk = 32;   // k per MFMA (16x16x32 BF16)
 constexpr int kMaxK3 = 384;  // K <= 128: A operands held in registers
 
-// phase[spin(s), w] += sum_k rec[k][s] * Bt[w][k] over one block of K3 =
+// phase[spin(s), w] += sum_k rec[s][k] * Bt[w][k] over one block of K3 =
 // 32 NKC rows, 16x16x32 BF16 MFMAs, split hi/lo (three products).
 // Workgroup: 4 waves and 64 slots; wave q owns slots [16q, 16q + 16) and
 // walks every waveform in tiles of 64.
 // - A (the block's displacements) is read from HBM once, straight into the
 //   registers that hold it as MFMA A operands for all waveform tiles (lane l:
-//   slot l % 16 of the wave, k = 8 (l / 16) .. + 8 of each 32-k chunk; 16
-//   lanes read 64 contiguous bytes of a record row).
+//   slot l % 16 of the wave, k = 8 (l / 16) .. + 8 of each 32-k chunk: 32
+//   contiguous bytes of the slot's record, the 4 lanes of a slot 128).
 // - B (the waveform block, a few hundred KB shared by all workgroups, served
 //   from L2) moves through two LDS buffers, one 32-k chunk of one waveform
 //   tile at a time: the next chunk's global load is in flight while the
@@ -106,7 +106,7 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
     uint16_t hb[8], lb[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const float v = row_ok ? rec[(int64_t)(kc * kChunk + kl + j) * ld + srow] : 0.0f;
+      const float v = row_ok ? rec[srow * ld + kc * kChunk + kl + j] : 0.0f;
       hb[j] = bf16_bits(v);
       lb[j] = bf16_bits(v - bf16_float(hb[j]));
     }
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) phase_gemm_fp32_kernel(
     __syncthreads();
     if (s < nslots) {
       for (int kk = 0; kk < kChunk && k0 + kk < K3; ++kk) {
-        const float x = rec[(int64_t)(k0 + kk) * ld + s];
+        const float x = rec[s * ld + k0 + kk];
 #pragma unroll
         for (int c = 0; c < 16; ++c) acc[c] = fmaf(x, sB[c][kk], acc[c]);
       }
@@ -291,8 +291,8 @@ int nz_phase_gemm(const float* rec, int64_t ld, int64_t nslots, const int32_t* s
                   int32_t K, const uint16_t* bt_hi, const uint16_t* bt_lo, const float* bt_f32,
                   int32_t W, int32_t mode, float* phase, void* stream) {
   if (!rec || !slot_spin || !phase) return fail(-1, "nz_phase_gemm: NULL buffer");
-  if (nslots <= 0 || ld < nslots || W <= 0 || K <= 0) return fail(-1, "nz_phase_gemm: bad sizes");
   const int32_t K3 = 3 * K;
+  if (nslots <= 0 || ld < K3 || W <= 0 || K <= 0) return fail(-1, "nz_phase_gemm: bad sizes");
   if (mode == 0) {
     if (!bt_hi || !bt_lo) return fail(-1, "nz_phase_gemm: MFMA mode needs bt_hi and bt_lo");
     if (K3 % kChunk || K3 > kMaxK3)

@@ -68,6 +68,7 @@ struct WalkArgs {
   uint32_t flags;
   int32_t xcd_chunk;  // slot blocks per XCD chunk (0: no remap)
   float* rec;         // displacement record (nz_walk_record) or null
+  int64_t rec_ld;     // floats per slot of the record
 };
 
 // Workgroups of a launch are dispatched to the 8 XCDs round-robin (block b
@@ -114,6 +115,16 @@ constexpr int kMomChunk = 256;
 #else
 #define NZ_WALK_ATTR
 #endif
+// The tracking walk (displacement record of the precision form, per-step
+// moments of the narrow-pulse driver) also holds the start position.
+#ifndef NZ_TRACK_WAVES_PER_EU
+#define NZ_TRACK_WAVES_PER_EU 3
+#endif
+#if NZ_TRACK_WAVES_PER_EU > 0
+#define NZ_TRACK_ATTR __attribute__((amdgpu_waves_per_eu(NZ_TRACK_WAVES_PER_EU)))
+#else
+#define NZ_TRACK_ATTR
+#endif
 
 // Per-launch parameters of one simulation's walk.
 // Step numbers are 32-bit on the device (the host keeps them below 2^32).
@@ -128,7 +139,8 @@ struct StepParams {
   const float* garea;
   double* stepmom;   // per-step moment sums (needs smom and target - mom_base <= kMomChunk)
   uint32_t flags;
-  float* rec;        // per-step displacement record [3 * (target - mom_base)][ld] or null
+  float* rec;        // per-step displacement record [slots][rec_ld] or null
+  int64_t rec_ld;    // floats per slot of the record (>= 3 * (target - mom_base))
 };
 
 // Walk the spin held in slot i (fields strided by ld) from its next step to
@@ -192,9 +204,12 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
       if (substep(g, s, q, P.seed, spin, step, P.dt, do_t2)) {
         step_end(g, s, G, Ga);
         if (kTrack && P.rec) {  // displacement since the start, for the blocked phase contraction
-          const int64_t r = (int64_t)(t - P.mom_base) * 3;
-#pragma unroll
-          for (int a = 0; a < 3; ++a) P.rec[(r + a) * ld + i] = (float)lane_disp(g, s, x0, a);
+          // slot-major: a lane's steps are contiguous (the lanes of a
+          // wavefront are at different steps, so rows of steps would scatter)
+          float* r = P.rec + i * P.rec_ld + (int64_t)(t - P.mom_base) * 3;
+          r[0] = (float)lane_disp(g, s, x0, 0);
+          r[1] = (float)lane_disp(g, s, x0, 1);
+          r[2] = (float)lane_disp(g, s, x0, 2);
         }
         if (kTrack && P.stepmom) {  // moment sums (sim3d_kernel.cu:191-237), FP64 throughout
           const int k = (int)(t - P.mom_base) * 6;
@@ -232,22 +247,31 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   if (st.nguard) st.nguard[i] += s.nguard;
 }
 
-template <bool kTrack>
-// The grid comes as a device copy (`gp` and `gpr` both point to it).  The
-// plain walk reads it through `gp`: its fields are then loaded where used
-// instead of all occupying scalar registers from kernel entry (fewer
-// spills).  The tracking walk stores to global memory in its loop, after which
-// loads through a plain pointer would be repeated; it reads through the
-// __restrict__ `gpr`, whose loads the compiler may keep.
-__global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A, const Grid* gp,
-                                                              const Grid* __restrict__ gpr) {
-  // per-step moment partials of the block (12 KiB at kMomChunk = 256)
-  __shared__ double smom[kTrack ? kMomChunk * 6 : 1];
+// The grid comes as a device copy.  The plain walk reads it through a plain
+// pointer: its fields are then loaded where used instead of all occupying
+// scalar registers from kernel entry (fewer spills).  The tracking walk
+// (displacement record, per-step moments) stores to global memory in its
+// loop, after which loads through a plain pointer would be repeated; it reads
+// through a __restrict__ pointer, whose loads the compiler may keep.
+__global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A, const Grid* gp) {
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
   const int64_t i = blk * blockDim.x + threadIdx.x;
   const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
-                     (uint32_t)A.mom_base, A.dt, A.gph, A.garea, A.stepmom, A.flags, A.rec};
-  walk_slot<kTrack>(kTrack ? *gpr : *gp, A.st, A.n, i, i < A.n, P, smom);
+                     (uint32_t)A.mom_base, A.dt, A.gph, A.garea, A.stepmom, A.flags, A.rec,
+                     A.rec_ld};
+  walk_slot<false>(*gp, A.st, A.n, i, i < A.n, P, nullptr);
+}
+
+__global__ void __launch_bounds__(256) NZ_TRACK_ATTR track_kernel(const WalkArgs A,
+                                                                const Grid* __restrict__ gpr) {
+  // per-step moment partials of the block (12 KiB at kMomChunk = 256)
+  __shared__ double smom[kMomChunk * 6];
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
+                     (uint32_t)A.mom_base, A.dt, A.gph, A.garea, A.stepmom, A.flags, A.rec,
+                     A.rec_ld};
+  walk_slot<true>(*gpr, A.st, A.n, i, i < A.n, P, smom);
 }
 
 // Absolute start positions (spin order) -> resident state (slot = spin).
@@ -1092,17 +1116,18 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
     a.flags = flags;
     a.xcd_chunk = w->xcd_chunk;
     a.rec = nullptr;
+    a.rec_ld = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (w->timing) {
       if (int rc = next_event_pair(w, &e0, &e1)) return rc;
       NZ_HIP(hipEventRecord(e0, s));
     }
     if (stepmom)
-      hipLaunchKernelGGL(walk_kernel<true>, dim3(blocks_for(w->n)), dim3(256), 0, s, a,
-                         (const Grid*)w->sub->g_dev, (const Grid*)w->sub->g_dev);
+      hipLaunchKernelGGL(track_kernel, dim3(blocks_for(w->n)), dim3(256), 0, s, a,
+                         (const Grid*)w->sub->g_dev);
     else
-      hipLaunchKernelGGL(walk_kernel<false>, dim3(blocks_for(w->n)), dim3(256), 0, s, a,
-                         (const Grid*)w->sub->g_dev, (const Grid*)w->sub->g_dev);
+      hipLaunchKernelGGL(walk_kernel, dim3(blocks_for(w->n)), dim3(256), 0, s, a,
+                         (const Grid*)w->sub->g_dev);
     NZ_HIP(hipGetLastError());
     if (w->timing) NZ_HIP(hipEventRecord(e1, s));
     target += chunk;
@@ -1113,12 +1138,13 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
 }
 
 int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, float* rec,
-                   int32_t* slot_spin, void* stream) {
+                   int64_t ld, int32_t* slot_spin, void* stream) {
   Walker* w = (Walker*)ww;
   if (int rc = check_walker(w, "nz_walk_record")) return rc;
   if (nsteps <= 0 || nsteps > 4096) return fail(-1, "nz_walk_record: nsteps must be in [1, 4096]");
   if (!(dt >= 0.0)) return fail(-1, "nz_walk_record: dt must be >= 0");
   if (!rec || !slot_spin) return fail(-1, "nz_walk_record: NULL rec or slot_spin");
+  if (ld < 3 * (int64_t)nsteps) return fail(-1, "nz_walk_record: ld must be at least 3 * nsteps");
   if (w->step + nsteps > (int64_t)UINT32_MAX) return fail(-1, "nz_walk_record: step counter overflow");
   hipStream_t s = (hipStream_t)stream;
   if (w->sort_interval > 0 && w->since_sort >= w->sort_interval)
@@ -1138,8 +1164,9 @@ int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, flo
   a.flags = 0;
   a.xcd_chunk = w->xcd_chunk;
   a.rec = rec;
-  hipLaunchKernelGGL(walk_kernel<true>, dim3(blocks_for(w->n)), dim3(256), 0, s, a,
-                     (const Grid*)w->sub->g_dev, (const Grid*)w->sub->g_dev);
+  a.rec_ld = ld;
+  hipLaunchKernelGGL(track_kernel, dim3(blocks_for(w->n)), dim3(256), 0, s, a,
+                     (const Grid*)w->sub->g_dev);
   NZ_HIP(hipGetLastError());
   NZ_HIP(hipMemcpyAsync(slot_spin, w->buf[w->cur].spin_id, sizeof(int32_t) * w->n,
                         hipMemcpyDeviceToDevice, s));

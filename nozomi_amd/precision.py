@@ -82,7 +82,8 @@ class PrecisionRun:
         self.walker = rt.Walker(sub, self.N, sort_interval=self.K)
         self.phase = torch.empty((self.N, self.W), dtype=torch.float32, device=d)
         nb = 2 if overlap else 1
-        self.rec = [torch.zeros((3 * self.K, self.N), dtype=torch.float32, device=d)
+        # slot-major displacement records [N][3K] (a lane's steps contiguous)
+        self.rec = [torch.zeros((self.N, 3 * self.K), dtype=torch.float32, device=d)
                     for _ in range(nb)]
         self.slot_spin = [torch.empty(self.N, dtype=torch.int32, device=d) for _ in range(nb)]
         self.bt_hi = [torch.empty((self.W, 3 * self.K), dtype=torch.int16, device=d)
@@ -125,7 +126,7 @@ class PrecisionRun:
             if self._used[b]:  # block idx - 2's contraction still reads these buffers
                 self.walk_stream.wait_event(self.gemm_done[b])
             check(L.nz_walk_record(self.walker.handle, self.seed & (2**64 - 1), k, self.dt,
-                                   _ptr(self.rec[b]), _ptr(self.slot_spin[b]), sa),
+                                   _ptr(self.rec[b]), 3 * self.K, _ptr(self.slot_spin[b]), sa),
                   "nz_walk_record")
             if record_hook is not None:
                 with torch.cuda.stream(self.walk_stream):
@@ -138,7 +139,7 @@ class PrecisionRun:
             check(L.nz_waveform_block(_ptr(self.g), self.T, _ptr(self.u), self.W, t0, self.K,
                                       _ptr(self.bt_hi[b]), _ptr(self.bt_lo[b]),
                                       _ptr(self.bt_f32[b]), sb), "nz_waveform_block")
-            check(L.nz_phase_gemm(_ptr(self.rec[b]), self.N, self.N, _ptr(self.slot_spin[b]),
+            check(L.nz_phase_gemm(_ptr(self.rec[b]), 3 * self.K, self.N, _ptr(self.slot_spin[b]),
                                   self.K, _ptr(self.bt_hi[b]), _ptr(self.bt_lo[b]),
                                   _ptr(self.bt_f32[b]), self.W, self.mode, _ptr(self.phase), sb),
                   "nz_phase_gemm")

@@ -19,14 +19,14 @@ DEV = torch.device("cuda:0")
 def test_gemm_modes_against_fp64(K):
     rng = np.random.default_rng(K)
     N, W = 1000, 200
-    rec = rng.normal(scale=5.0, size=(3 * K, N)).astype(np.float32)
+    rec = rng.normal(scale=5.0, size=(N, 3 * K)).astype(np.float32)  # slot-major
     bt = rng.normal(scale=1e-3, size=(W, 3 * K)).astype(np.float32)
     slot_spin = rng.permutation(N).astype(np.int32)
     slot_spin[::7] = -1                      # padding slots contribute nothing
     ph0 = rng.normal(size=(N, W)).astype(np.float32)
     ref = ph0.astype(np.float64).copy()
     valid = slot_spin >= 0
-    ref[slot_spin[valid]] += rec[:, valid].T.astype(np.float64) @ bt.T.astype(np.float64)
+    ref[slot_spin[valid]] += rec[valid].astype(np.float64) @ bt.T.astype(np.float64)
     # BF16 hi/lo of bt the way the library splits (round to nearest even)
     u = bt.view(np.uint32).astype(np.uint64)
     hi = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
@@ -35,10 +35,10 @@ def test_gemm_modes_against_fp64(K):
     lo = ((ul + 0x7FFF + ((ul >> 16) & 1)) >> 16).astype(np.uint16)
     t = {k: torch.from_numpy(v).to(DEV) for k, v in
          dict(rec=rec, ss=slot_spin, hi=hi.view(np.int16), lo=lo.view(np.int16), f32=bt).items()}
-    scale = np.abs(rec).T.astype(np.float64) @ np.abs(bt).T.astype(np.float64)
+    scale = np.abs(rec).astype(np.float64) @ np.abs(bt).T.astype(np.float64)
     for mode, tol in ((0, 6e-5), (1, 1e-5)):
         ph = torch.from_numpy(ph0.copy()).to(DEV)
-        check(lib().nz_phase_gemm(_ptr(t["rec"]), N, N, _ptr(t["ss"]), K, _ptr(t["hi"]), _ptr(t["lo"]),
+        check(lib().nz_phase_gemm(_ptr(t["rec"]), 3 * K, N, _ptr(t["ss"]), K, _ptr(t["hi"]), _ptr(t["lo"]),
                                   _ptr(t["f32"]), W, mode, _ptr(ph), _stream_ptr(DEV)))
         got = ph.cpu().numpy().astype(np.float64)
         err = np.abs(got - ref)
@@ -59,7 +59,7 @@ def test_precision_walk_against_oracle(oracle_lib):
     blocks = []
 
     def hook(t0, k, rec, slot_spin):
-        blocks.append((t0, k, rec[:3 * k].cpu().numpy().copy(), slot_spin.cpu().numpy().copy()))
+        blocks.append((t0, k, rec[:, :3 * k].cpu().numpy().copy(), slot_spin.cpu().numpy().copy()))
 
     runs = {}
     for mode in ("mfma", "fp32"):
@@ -73,7 +73,7 @@ def test_precision_walk_against_oracle(oracle_lib):
     g64, u64 = g.astype(np.float64), u.astype(np.float64)
     ph_rec = (x0h.T @ u64) * g64.sum(1)[None, :]
     for t0, k, rec, ss in blocks:
-        d = rec.reshape(k, 3, N)                                   # [t, a, slot]
+        d = rec.reshape(N, k, 3).transpose(1, 2, 0)                # [t, a, slot]
         contrib = np.einsum("tas,wt,aw->sw", d, g64[:, t0:t0 + k], u64)
         ph_rec[ss] += contrib
     for mode in ("mfma", "fp32"):
@@ -137,7 +137,7 @@ def test_precision_form_full_size_two_blocks():
         slot_of = torch.empty(N, dtype=torch.int64, device=DEV)
         slot_of[slot_spin.long()] = torch.arange(N, device=DEV)
         cols = slot_of[pick]
-        blocks.append((t0, k, rec[:3 * k].index_select(1, cols).cpu().numpy().astype(np.float64)))
+        blocks.append((t0, k, rec.index_select(0, cols)[:, :3 * k].cpu().numpy().astype(np.float64)))
 
     print(f"precision form: {N} spins x {T} steps, {W} waveforms, K = {K}", flush=True)
     pr = PrecisionRun(sub, x0, seed, g, u, dt, K=K, mode="mfma")
@@ -152,7 +152,7 @@ def test_precision_form_full_size_two_blocks():
     ref = (x0h.T @ u64) * g64.sum(1)[None, :]
     scale = (np.abs(x0h).T @ np.abs(u64)) * np.abs(g64).sum(1)[None, :]
     for t0, k, d in blocks:
-        d = d.reshape(k, 3, -1)
+        d = d.reshape(-1, k, 3).transpose(1, 2, 0)  # [t, a, spin of the subsample]
         ref += np.einsum("tas,wt,aw->sw", d, g64[:, t0:t0 + k], u64)
         scale += np.einsum("tas,wt,aw->sw", np.abs(d), np.abs(g64[:, t0:t0 + k]), np.abs(u64))
     err = np.abs(ph - ref)
