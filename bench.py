@@ -470,17 +470,28 @@ def batching_form(args):
 def serving_form(args):
     from nozomi_amd.serve import (Server, fifo_latencies, latency_stats, run_serial, serve_trace)
     from nozomi_amd.workloads import pregenerate, serving_mix, total_spin_steps
+    from nozomi_amd.serve import SubstrateCache
+    import torch
+    dev = torch.device("cuda:0")
     # serving: Poisson arrivals at 50 req/s, continuous batching, HBM admission;
-    # device substrates are built when a request first needs one
+    # device substrates are built when a request first needs one.  The
+    # substrate families are tuned before the trace (deployment time); the
+    # served and the serial leg each start from an empty substrate cache and
+    # build what their requests need.
     reqs, arr = serving_mix(rate=50.0, duration=args.serve_seconds, seed=args.seed + 1,
                             scale_steps=args.forms_steps)
     pregenerate(reqs)
+    t0 = time.perf_counter()
+    families = SubstrateCache(dev, 0, tune=True).pretune(reqs)
+    t_tune = time.perf_counter() - t0
+    run_serial(reqs[:2], cache=SubstrateCache(dev, 1 << 40, tune=True))  # warms the kernels
     cap = 16 * 1024 * 1024
     srv = Server(cap, hbm_budget=64 << 30, max_sims=256, k=64, tune=True)
     got, makespan = serve_trace(srv, reqs, arr, progress=_progress("served"))
     built, reused = srv.subs.built, srv.subs.reused
     srv.close()
-    alone, t_s = run_serial(reqs, cache=srv.subs, progress=_progress("serial"))
+    alone, t_s = run_serial(reqs, cache=SubstrateCache(dev, 64 << 30, tune=True),
+                            progress=_progress("serial"))
     diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
     ss = total_spin_steps(reqs)
     lat_serial = fifo_latencies(arr, [alone[r.rid].latency for r in reqs])
@@ -493,6 +504,7 @@ def serving_form(args):
         "serial_fifo_latency": {"p50_s": float(np.percentile(lat_serial, 50)),
                                 "p99_s": float(np.percentile(lat_serial, 99))},
         "substrates_built": built, "substrates_reused": reused,
+        "families_tuned": len(families), "tuning_s_before_trace": t_tune,
         "max_signal_diff_vs_alone": diff,
     }
 

@@ -53,14 +53,16 @@ def tune(geom, key=None, n=65536, steps=128, dt=0.01, cells=(1.5, 1.75, 2.0, 2.2
 
 
 def family_key(geom):
-    """Substrates whose sphere density (spheres per um^3, in steps of 2^0.25)
-    and mean sphere radius (in steps of 0.1 um) agree share their tuning."""
+    """Substrates whose sphere density (spheres per um^3, in factors of 2) and
+    mean sphere radius (in steps of 0.25 um) agree share their tuning.  Finer
+    classes buy nothing: on the beaded-axon mixes the choices of neighbouring
+    classes differ by less than the run-to-run spread of the timings."""
     sp = np.asarray(geom.spheres)
     if sp.shape[1] == 0:
         return ("empty",)
     vol = float(geom.Lx) * float(geom.Ly) * float(geom.Lz)
     dens = sp.shape[1] / vol
-    return (int(round(4 * np.log2(dens))), int(round(10 * float(sp[3].mean()))))
+    return (int(round(np.log2(dens))), int(round(4 * float(sp[3].mean()))))
 
 
 def tuned(geom, device=None, **kw):

@@ -126,12 +126,6 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
         err = "candidate lists exceed 2^26 entries (2 GiB of records); lower cell_size or margin";
         return -1;
       }
-      for (int64_t c = 0; c < ncells; ++c)
-        if (cnt[c + 1] - cnt[c] > kMaxListLen) {
-          err = "a cell lists more than " + std::to_string(kMaxListLen) +
-                " spheres; lower cell_size or margin";
-          return -1;
-        }
       // padding records after the last list: the scan loads ahead, up to
       // kCandPad - 1 records past the end of a list
       out.cand.assign(2 * ((size_t)cnt[ncells] + kCandPad), make_float4(0, 0, 0, 0));

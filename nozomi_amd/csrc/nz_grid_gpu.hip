@@ -119,7 +119,7 @@ __global__ void grid_records_kernel(const GridGeo G, const int32_t* start, int64
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const int32_t b = start[c], e = start[c + 1];
-  for (int32_t k = b + 1; k < e; ++k) {  // insertion sort (lists hold at most 255)
+  for (int32_t k = b + 1; k < e; ++k) {  // insertion sort (lists are short)
     const int32_t v = ent[k];
     int32_t j = k - 1;
     while (j >= b && ent[j] > v) {
@@ -261,9 +261,6 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
   if (hs[1] > (unsigned long long)kMaxCandEntries)
     return fail(-1, "nz_substrate_create: candidate lists exceed 2^26 entries (2 GiB of records); "
                     "lower cell_size or margin");
-  if (hs[0] > (unsigned long long)kMaxListLen)
-    return fail(-1, "nz_substrate_create: a cell lists more than " + std::to_string(kMaxListLen) +
-                        " spheres; lower cell_size or margin");
   const int64_t total = (int64_t)hs[1];
 
   // 2. CSR starts (kept), 3. lists, 4. records

@@ -15,7 +15,8 @@ namespace nz {
 constexpr float kTol = 1e-4f;
 constexpr float kLarge = 1e6f;
 constexpr int kMaxIter = 100;
-// Longest candidate list of a cell (the FP32 scan keys a list position in 8 bits).
+// Longest candidate list the FP32 scan takes (it keys a list position in 8
+// bits); a sub-step on a cell with a longer list scans it in FP64.
 constexpr int kMaxListLen = 255;
 // Rejection-seeding attempts per spin before it counts as unplaceable: a
 // target filling 0.2% of the box (a 0.5 um cylinder in a 20 um box) needs
@@ -53,7 +54,7 @@ struct Grid {
   // record cand[2e] = a, cand[2e+1] = b with cell-relative coordinates:
   //   a = (cx - lo_x, cy - lo_y, cz - lo_z, (r+tol)^2)
   //   b = (r^2, (r-tol)^2, sid, sphere index as int bits)
-  // A list holds at most 255 entries.
+  // A list longer than kMaxListLen is scanned in FP64.
   const int32_t* cell_start;  // [ncells + 1]
   // sort position of each in-box cell (index (z ncell_y + y) ncell_x + x):
   // cells grouped in 4x4x4 regions, ordered by list length within a region,

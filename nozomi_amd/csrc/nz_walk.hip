@@ -487,8 +487,10 @@ int exclusive_scan_i32(int32_t* counts, int64_t m, int32_t* partial, hipStream_t
   return 0;
 }
 
+// Slots [0, n) of src (fields strided by ld) to their sorted places in dst.
 __global__ void __launch_bounds__(256) sort_scatter_kernel(const SlotState src, const SlotState dst,
-                                                           int64_t n, const int32_t* keys,
+                                                           int64_t n, int64_t ld,
+                                                           const int32_t* keys,
                                                            const int32_t* offsets, int32_t* cursor) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
@@ -503,11 +505,11 @@ __global__ void __launch_bounds__(256) sort_scatter_kernel(const SlotState src, 
   dst.spin_id[j] = src.spin_id[i];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    dst.cell[a * n + j] = src.cell[a * n + i];
-    dst.x[a * n + j] = src.x[a * n + i];
-    dst.wrap[a * n + j] = src.wrap[a * n + i];
-    dst.x0[a * n + j] = src.x0[a * n + i];
-    dst.mom[a * n + j] = src.mom[a * n + i];
+    dst.cell[a * ld + j] = src.cell[a * ld + i];
+    dst.x[a * ld + j] = src.x[a * ld + i];
+    dst.wrap[a * ld + j] = src.wrap[a * ld + i];
+    dst.x0[a * ld + j] = src.x0[a * ld + i];
+    dst.mom[a * ld + j] = src.mom[a * ld + i];
   }
   dst.sid[j] = src.sid[i];
   dst.logsig[j] = src.logsig[i];
@@ -636,7 +638,7 @@ static int sort_spins(Walker* w, hipStream_t stream) {
   hipLaunchKernelGGL(tile_scan_kernel, dim3(nb), dim3(kScanThreads), 0, stream, w->counts, m,
                      w->partial, (const int64_t*)nullptr);
   hipLaunchKernelGGL(sort_scatter_kernel, dim3(blocks_for(n)), dim3(256), 0, stream,
-                     w->buf[w->cur], w->buf[1 - w->cur], n, w->keys, w->counts, w->cursor);
+                     w->buf[w->cur], w->buf[1 - w->cur], n, n, w->keys, w->counts, w->cursor);
   NZ_HIP(hipGetLastError());
   w->cur = 1 - w->cur;
   w->since_sort = 0;
@@ -686,6 +688,8 @@ struct BatchJob {
 
 struct Batch {
   int64_t cap = 0, nblocks = 0;
+  int64_t active_blocks = 0;   // blocks up to the last one a simulation owns
+  int64_t launch_blocks = 64;  // blocks a launch covers (>= active_blocks)
   int32_t max_sims = 0;
   SlotState buf[2];
   int cur = 0;
@@ -755,9 +759,10 @@ __global__ void __launch_bounds__(256) batch_keys_kernel(const SimDesc* descs,
                                                          const int32_t* block_sim,
                                                          const int64_t* block_key,
                                                          const SlotState st, int64_t ld,
-                                                         int32_t* keys, int32_t* counts) {
+                                                         int64_t nslots, int32_t* keys,
+                                                         int32_t* counts) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ld) return;
+  if (i >= nslots) return;
   const int64_t pb = i / kBlk;
   const int32_t s = block_sim[pb];
   int64_t key;
@@ -830,6 +835,7 @@ static void batch_release(Batch* b) {
 static int64_t batch_layout(Batch* b) {
   int64_t key = 0;
   int64_t pb = 0;
+  b->active_blocks = 0;
   while (pb < b->nblocks) {
     const int32_t s = b->block_sim[pb];
     if (s >= 0) {
@@ -838,6 +844,7 @@ static int64_t batch_layout(Batch* b) {
       key += 2 * d.ncells + 1;
       const int64_t e = pb + b->jobs[s].nb;
       for (; pb < e; ++pb) b->block_key[pb] = d.key0;
+      b->active_blocks = pb;
     } else {
       for (; pb < b->nblocks && b->block_sim[pb] < 0; ++pb) b->block_key[pb] = key;
       ++key;
@@ -869,6 +876,23 @@ static int batch_ensure_keys(Batch* b, int64_t nkeys) {
   return 0;
 }
 
+// Blocks a launch covers: a power of two (at least 64) at or above the last
+// block a simulation owns, grown as soon as a simulation lands beyond it and
+// shrunk when the used range falls to a quarter of it.  A change drops the
+// captured graph; callers hold the stream synchronized.
+static void batch_fit_launch(Batch* b) {
+  int64_t want = 64;
+  while (want < b->active_blocks) want <<= 1;
+  want = std::min(want, b->nblocks);
+  if (b->active_blocks > b->launch_blocks || want * 4 <= b->launch_blocks) {
+    if (want != b->launch_blocks && b->graph) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
+    b->launch_blocks = want;
+  }
+}
+
 // Bring the device's progress fields to the host mirror (before the host
 // rewrites the tables) and upload the tables when they changed.
 static int batch_sync_tables(Batch* b, bool upload) {
@@ -882,6 +906,7 @@ static int batch_sync_tables(Batch* b, bool upload) {
   if (!upload) return 0;
   const int64_t nkeys = batch_layout(b);
   if (int rc = batch_ensure_keys(b, nkeys)) return rc;
+  batch_fit_launch(b);
   NZ_HIP(hipMemcpy(b->descs_dev, b->descs.data(), sizeof(SimDesc) * b->max_sims,
                    hipMemcpyHostToDevice));
   NZ_HIP(hipMemcpy(b->block_sim_dev, b->block_sim.data(), sizeof(int32_t) * b->nblocks,
@@ -903,9 +928,12 @@ static int batch_enqueue_launch(Batch* b) {
                      (const int64_t*)b->nkeys_dev, (int64_t)1, 0);
   hipLaunchKernelGGL(fill_i32_dev, dim3(blocks_for(mk)), dim3(256), 0, s, b->cursor,
                      (const int64_t*)b->nkeys_dev, (int64_t)0, 0);
-  hipLaunchKernelGGL(batch_keys_kernel, dim3(blocks_for(b->cap)), dim3(256), 0, s,
+  // only the blocks up to the last used one: the pool's tail is free and
+  // keeps its place (its keys come last in pool order)
+  const int64_t nslots = b->launch_blocks * kBlk;
+  hipLaunchKernelGGL(batch_keys_kernel, dim3(blocks_for(nslots)), dim3(256), 0, s,
                      (const SimDesc*)b->descs_dev, (const int32_t*)b->block_sim_dev,
-                     (const int64_t*)b->block_key_dev, src, b->cap, b->keys, b->counts);
+                     (const int64_t*)b->block_key_dev, src, b->cap, nslots, b->keys, b->counts);
   const int64_t nbt = (mk + 1 + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(tile_sum_kernel, dim3(nbt), dim3(kScanThreads), 0, s, b->counts, mk + 1,
                      b->partial, (const int64_t*)b->nkeys_dev);
@@ -913,9 +941,9 @@ static int batch_enqueue_launch(Batch* b) {
                      (const int64_t*)b->nkeys_dev);
   hipLaunchKernelGGL(tile_scan_kernel, dim3(nbt), dim3(kScanThreads), 0, s, b->counts, mk + 1,
                      (const int32_t*)b->partial, (const int64_t*)b->nkeys_dev);
-  hipLaunchKernelGGL(sort_scatter_kernel, dim3(blocks_for(b->cap)), dim3(256), 0, s, src, dst,
-                     b->cap, (const int32_t*)b->keys, (const int32_t*)b->counts, b->cursor);
-  hipLaunchKernelGGL(batch_walk_kernel, dim3((unsigned)b->nblocks), dim3(256), 0, s,
+  hipLaunchKernelGGL(sort_scatter_kernel, dim3(blocks_for(nslots)), dim3(256), 0, s, src, dst,
+                     nslots, b->cap, (const int32_t*)b->keys, (const int32_t*)b->counts, b->cursor);
+  hipLaunchKernelGGL(batch_walk_kernel, dim3((unsigned)b->launch_blocks), dim3(256), 0, s,
                      (const SimDesc*)b->descs_dev, (const int32_t*)b->block_sim_dev, dst, b->cap,
                      b->xcd_chunk);
   hipLaunchKernelGGL(batch_advance_kernel, dim3(blocks_for(b->max_sims)), dim3(256), 0, s,
@@ -1255,6 +1283,7 @@ int nz_batch_create(int64_t capacity, int32_t max_sims, nz_batch_t* out) {
   Batch* b = new Batch();
   b->nblocks = (capacity + kBlk - 1) / kBlk;
   b->cap = b->nblocks * kBlk;
+  b->launch_blocks = std::min<int64_t>(64, b->nblocks);
   b->max_sims = max_sims;
   (void)hipGetDevice(&b->device);
   if (const char* e = getenv("NZ_WALK_XCD_CHUNK")) b->xcd_chunk = std::max(0, atoi(e));
@@ -1399,6 +1428,7 @@ int nz_batch_add(nz_batch_t bb, nz_substrate_t sub, int64_t n, uint64_t seed, in
   const int64_t nkeys = batch_layout(b);
   if (int rc = batch_ensure_keys(b, nkeys)) return rc;
   NZ_HIP(hipStreamSynchronize(b->stream));
+  batch_fit_launch(b);
   NZ_HIP(hipMemcpy(b->descs_dev, b->descs.data(), sizeof(SimDesc) * b->max_sims, hipMemcpyHostToDevice));
   NZ_HIP(hipMemcpy(b->block_sim_dev, b->block_sim.data(), sizeof(int32_t) * b->nblocks,
                    hipMemcpyHostToDevice));

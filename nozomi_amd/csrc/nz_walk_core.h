@@ -691,6 +691,7 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
       }
     }
     acc_reset(g, acc);
+    if (exact_mode < 0 && q.ce - q.cs > kMaxListLen) exact_mode = kExactList;  // dense cell
     if (exact_mode < 0) {
       ScanIn in;
 #pragma unroll

@@ -124,6 +124,24 @@ class SubstrateCache:
             self.acquire(r.key, r.make_geometry)
             self.release(r.key)
 
+    def pretune(self, requests):
+        """Tunes every substrate family the requests use, ahead of serving
+        (the autotuner keeps one choice per family for the process), so that
+        no request waits for a tuning sweep.  Builds no device substrate.
+        Returns {family key: (cell, margin, k)}."""
+        from .autotune import family_key, tuned
+        out, seen = {}, set()
+        for r in requests:
+            if r.key in seen:
+                continue
+            seen.add(r.key)
+            geom = r.make_geometry()
+            fk = family_key(geom)
+            if fk not in out:
+                p = tuned(geom, device=self.device)
+                out[fk] = (p["cell"], p["margin"], p["k"])
+        return out
+
     def release(self, key):
         if key in self.items:
             self.items[key][3] -= 1

@@ -125,6 +125,10 @@ class DiffSim3d:
         self._sig = torch.empty((1, self.nspins), dtype=f32, device=self.device)
         self._mom = torch.empty((3, self.nspins), dtype=f32, device=self.device)
         self._fresh = False
+        # which reference kernel the phase buffer belongs to: "axes" after
+        # dwi_step / simulate (randomWalk3d_phase writes x, y, z phases),
+        # "directions" after dwi_multidirections_step (one row per diffdir)
+        self._phase_kind = "directions"
         self.issetup = True
         self.spins = self.spins_d.get()
 
@@ -157,10 +161,18 @@ class DiffSim3d:
 
     @property
     def phase_d(self):
-        """Phases [ndiffdir * N] of the multi-direction kernel: the projection
-        of the moment on every direction (diffsim3d.py:45, sim3d_kernel.cu:880-942)."""
+        """The reference's phase buffer, ndiffdir * N values (diffsim3d.py:45).
+        After dwi_multidirections_step: the phase of every direction of
+        diffdir, the projection of the moment (sim3d_kernel.cu:880-942).  After
+        dwi_step: randomWalk3d_phase's x, y, z phases in the first three rows
+        (sim3d_kernel.cu:674-715), the rows beyond them zero as allocated."""
         self._views()
-        ph = rt.project_phase(self._mom, self.diffdir)
+        if self._phase_kind == "axes":
+            ph = torch.zeros((self.ndiffdir, self.nspins), dtype=torch.float32, device=self.device)
+            k = min(3, self.ndiffdir)
+            ph[:k] = self._mom[:k]
+        else:
+            ph = rt.project_phase(self._mom, self.diffdir)
         return DeviceArray(ph.reshape(-1))
 
     # -- stepping (diffsim3d.py:180-226) ---------------------------------------
@@ -188,12 +200,16 @@ class DiffSim3d:
         self._walk(1, gwave_dt, g=g, garea=ga, t2=False)
 
     def dwi_step(self, gwave_dt, G, G_area_at_each_time_step):
-        """One step accumulating x.G per axis (randomWalk3d_phase)."""
+        """One step accumulating x.G per axis (randomWalk3d_phase); phase_d
+        then holds the axis phases."""
         self._phase_step(gwave_dt, G, G_area_at_each_time_step)
+        self._phase_kind = "axes"
 
     def dwi_multidirections_step(self, gwave_dt, G, G_area_at_each_time_step):
-        """One step accumulating (x.u_d)G per direction (randomWalk3d_phase_multidirr)."""
+        """One step accumulating (x.u_d)G per direction (randomWalk3d_phase_multidirr);
+        phase_d then holds the direction phases."""
         self._phase_step(gwave_dt, G, G_area_at_each_time_step)
+        self._phase_kind = "directions"
 
     def calculate_diffusion_coefficients_and_kurtoses(self, step_idx, current_time,
                                                       dx_array, dy_array, dz_array,
@@ -227,21 +243,25 @@ class DwiSim3d(DiffSim3d):
         g, ga = wf.phase_coefficients(gwave.wave, gwave.dt)
         return (torch.from_numpy(g).to(self.device), torch.from_numpy(ga).to(self.device))
 
-    def _simulate(self, gwave, structures, initstates):
+    def _simulate(self, gwave, structures, initstates, kind):
         if not self.issetup:
             self.setup(structures=structures, initstates_int=initstates)
         g, ga = self._coefficients(gwave)
         self._walk(len(gwave.wave), gwave.dt, g=g, garea=ga, t2=False)
+        self._phase_kind = kind
 
     def simulate(self, gwave, structures=None, initstates=None):
-        """Phases along x, y, z: returns numpy [3, N] (diffsim3d.py:230-244)."""
-        self._simulate(gwave, structures, initstates)
-        return self.mom_d.get().reshape(3, self.nspins)
+        """Phases along x, y, z (the dwi_step kernel), returned as the
+        reference returns its phase buffer: numpy [ndiffdir, N], the x, y, z
+        phases in the first three rows (diffsim3d.py:230-244; with the default
+        diffdir that is [3, N])."""
+        self._simulate(gwave, structures, initstates, "axes")
+        return self.phase_d.get().reshape(self.ndiffdir, self.nspins)
 
     def simulate_multi_directions(self, gwave, structures=None, initstates=None):
         """Phases along every direction of diffdir: numpy [ndiffdir, N]
         (diffsim3d.py:246-261)."""
-        self._simulate(gwave, structures, initstates)
+        self._simulate(gwave, structures, initstates, "directions")
         return self.phase_d.get().reshape(self.ndiffdir, self.nspins)
 
     def signals(self, scales=None):

@@ -53,11 +53,34 @@ def test_device_grid_equals_host_grid(name):
     assert np.array_equal(np.sort(rank), np.arange(rank.size))
 
 
-def test_list_overflow_is_reported():
-    # 300 concentric spheres: one cell would list all of them (> 255)
-    sp = np.zeros((5, 300))
-    sp[3] = np.linspace(0.5, 1.5, 300)
-    from nozomi_amd.runtime import DeviceSubstrate
-    with pytest.raises(ValueError, match="more than 255"):
-        DeviceSubstrate(sp, 1, [2.0, 2.0], [100.0, 100.0], (10.0, 10.0, 10.0), cell_size=1.0,
-                        margin=0.8, device=DEV)
+def test_dense_cell_lists_walk_in_fp64(oracle_lib):
+    """300 concentric spheres of one structure: a cell lists all of them,
+    beyond the 255 entries the FP32 scan keys; such a cell is scanned in FP64
+    and the walk still follows the oracle."""
+    from nozomi_amd.simulation_engine.diffsim3d import DiffSim3d
+    from nozomi_amd.simulation_engine.geometry import SimGeometry3D, Structure3D
+    g = SimGeometry3D(10.0, 10.0, 10.0, 2.0, 100.0, 1)
+    r = np.linspace(0.5, 1.5, 300)
+    z = np.zeros_like(r)
+    g.add_structure(Structure3D(z, z, z, r, 2.0, 100.0, 1))
+    N, T, dt = 2048, 100, 0.02
+    sim = DiffSim3d(g, N, cell_size=1.0, margin=0.8, counters=True)
+    sim.setup(initstates_int=21)
+    assert sim.sub.info()["ncand_max"] > 255
+    p0 = sim.spins_d.get().astype(np.float64)
+    sim.run(T, dt)
+    gp, gd = sim.walker.read_f64()
+    _, nguard = sim.walker.counters()
+    orc = oracle_lib.OracleSubstrate(g.spheres, g.nstructures, g.structure_D(), g.structure_T2(),
+                                     (10.0, 10.0, 10.0))
+    pos, pos0 = p0.copy(), p0.copy()
+    orc.walk(pos, pos0, np.ones(N), seed=21, step0=0, nsteps=T, dt=dt)
+    d = gp - pos
+    err = np.abs(d - 10.0 * np.round(d / 10.0)).max(axis=0)
+    print(f"dense cells: FP64 scans {int(nguard.sum())}, within 1e-5 {np.mean(err < 1e-5):.4f}, "
+          f"bit-identical {np.mean(err == 0):.4f}", flush=True)
+    assert nguard.sum() > 1000
+    assert np.mean(err < 1e-5) >= 0.999
+    # the union of the spheres confines the spins seeded inside it
+    inside0 = np.sqrt((p0 ** 2).sum(0)) < 1.5
+    assert (np.sqrt((gp[:, inside0] ** 2).sum(0)) < 1.5).all()

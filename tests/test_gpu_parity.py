@@ -368,3 +368,52 @@ def test_step_moments_free_diffusion_match_oracle(oracle_lib):
     rel = np.abs(got / ref - 1).max()
     print(f"free step moments: max rel diff {rel:.2e}")
     assert rel < 1e-10
+
+
+def test_dwi_step_apis_and_phase_buffer(c0_geometry, oracle_lib):
+    """DwiSim3d.simulate (the reference's randomWalk3d_phase path) returns the
+    phase buffer [ndiffdir, N] with the x, y, z phases in its first three rows
+    and zeros after them, also for a non-default diffdir (diffsim3d.py:230-244);
+    simulate_multi_directions returns one row per direction; the per-step API
+    (dwi_step / dwi_multidirections_step, one launch per step) equals the
+    whole-waveform walk bit for bit; the axis phases equal the oracle's."""
+    g = c0_geometry
+    prot = PGSEShells(2.0, 4.0, 0.02, [1.0], icosahedron_6())  # 300 steps, 6 directions
+    N = 4096
+
+    class Wave:
+        wave = prot.wave
+        dt = prot.dt
+
+    def fresh():
+        sim = DwiSim3d(g, N, diffdir=prot.dirs)
+        sim.setup(initstates_int=55)
+        return sim
+
+    a = fresh()
+    p0 = a.spins_d.get().astype(np.float64)
+    ph_axes = a.simulate(Wave())
+    assert ph_axes.shape == (6, N)
+    assert np.all(ph_axes[3:] == 0)
+    b = fresh()
+    ph_dirs = b.simulate_multi_directions(Wave())
+    assert ph_dirs.shape == (6, N)
+    assert np.allclose(ph_dirs, prot.dirs.T @ ph_axes[:3].astype(np.float64), atol=1e-5)
+    # the per-step API, one launch per step
+    from nozomi_amd.simulation_engine import waveforms as wf
+    G, Ga = wf.phase_coefficients(prot.wave, prot.dt)
+    c = fresh()
+    for n in range(prot.nsteps):
+        c.dwi_step(prot.dt, G[n], Ga[n])
+    assert np.array_equal(c.phase_d.get().reshape(6, N), ph_axes)
+    d = fresh()
+    for n in range(prot.nsteps):
+        d.dwi_multidirections_step(prot.dt, G[n], Ga[n])
+    assert np.array_equal(d.phase_d.get().reshape(6, N), ph_dirs)
+    # the axis phases against the oracle's walk of the same spins
+    orc = _oracle_for(g, oracle_lib)
+    pos, pos0, sig, mom = p0.copy(), p0.copy(), np.ones(N), np.zeros((3, N))
+    orc.walk(pos, pos0, sig, seed=55, step0=0, nsteps=prot.nsteps, dt=prot.dt,
+             g=G.astype(np.float64), garea=Ga.astype(np.float64), mom=mom)
+    print(f"axis phases vs oracle: max {np.abs(ph_axes[:3] - mom).max():.2e}")
+    assert np.abs(ph_axes[:3] - mom).max() < 1e-4
