@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define NZ_ABI_VERSION 3
+#define NZ_ABI_VERSION 4
 
 typedef struct nz_substrate* nz_substrate_t;
 
@@ -88,6 +88,20 @@ int nz_substrate_info(nz_substrate_t sub, nz_substrate_info_t* info);
  *                Synchronous. */
 int nz_substrate_export_grid(nz_substrate_t sub, int32_t* cell_start, int32_t* cell_rank,
                              float* cand);
+
+/* Copy a substrate's reach entries to host memory (inspection and tests; no
+ * counterpart in the reference).  The candidate scan's first pass reads one
+ * 8-byte entry per list entry: binary16 (h_x, h_y, h_z, k) with h = -2 sigma
+ * c' (c' the centre relative to the cell's lower corner as h encodes it) and
+ * k at most sigma (|c'|^2 - (1 + lambda) R'^2), R' = r + tol + |c - c'|.
+ * Each pointer may be NULL:
+ *   nentries    : int64, the number of entries (lists padded to even
+ *                 lengths, then 16 zero entries)
+ *   reach_start : int32 [nloc_x nloc_y nloc_z], first entry of each cell's list
+ *   reach       : uint32 [2 nentries], h_x | h_y << 16 and h_z | k << 16.
+ * Synchronous. */
+int nz_substrate_export_reach(nz_substrate_t sub, int64_t* nentries, int32_t* reach_start,
+                              uint32_t* reach);
 
 /* Rejection-seed n spins uniformly inside the target structures.
  * Replaces SimGeometry3D.gpu_seed + seedSpinsKernel (geometry.py:73-174,

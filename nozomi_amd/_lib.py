@@ -9,7 +9,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libnozomi_hip.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 NZ_WALK_T2 = 1
 NZ_WALKER_COUNTERS = 1
@@ -51,6 +51,7 @@ _SIGS = {
     "nz_substrate_destroy": (ctypes.c_int, [_c_vp]),
     "nz_substrate_info": (ctypes.c_int, [_c_vp, ctypes.POINTER(NzSubstrateInfo)]),
     "nz_substrate_export_grid": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp]),
+    "nz_substrate_export_reach": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp]),
     "nz_seed": (ctypes.c_int, [_c_vp, ctypes.c_int64, _c_vp, ctypes.c_int32, ctypes.c_uint64,
                                ctypes.c_int32, _c_vp, _c_vp, _c_vp]),
     "nz_walker_create": (ctypes.c_int, [_c_vp, ctypes.c_int64, ctypes.c_uint32,

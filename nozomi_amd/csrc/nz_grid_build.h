@@ -31,6 +31,8 @@ struct HostGrid {
   std::vector<int32_t> start;
   std::vector<int32_t> rank;
   std::vector<float4> cand;  // a, b records interleaved
+  std::vector<uint2> reach;
+  std::vector<int32_t> reach_start;
   std::vector<double4> sph;
   std::vector<int32_t> sid;
   std::vector<double> D, T2;
@@ -42,6 +44,9 @@ struct HostGrid {
     g.cell_rank = rank.data();
     g.cand = cand.data();
     g.cand_bytes = (int32_t)(sizeof(float4) * cand.size());
+    g.reach = reach.data();
+    g.reach_start = reach_start.data();
+    g.reach_bytes = (int32_t)(sizeof(uint2) * reach.size());
     g.sph = sph.data();
     g.sph_sid = sid.data();
     g.D = D.data();
@@ -90,7 +95,115 @@ inline int grid_params(const double* spheres, int64_t S, int32_t nstruct, const 
     hd[a] = L[a] / g.ncell[a];
   }
   M = margin + 4.0 * (double)kTol + 1e-5;
+  // reach test parameters.  lambda = ell / R makes the bound exact; a
+  // sub-step's segment is typically 0.4 of the list margin (the margin is
+  // sized for the longer jumps), R the median radius + tol.
+  double rmax = 0.0;
+  std::vector<double> radii(spheres + 3 * S, spheres + 4 * S);
+  for (double r : radii) rmax = std::max(rmax, r);
+  double rmed = 1.0;
+  if (S > 0) {
+    std::nth_element(radii.begin(), radii.begin() + S / 2, radii.end());
+    rmed = radii[S / 2];
+  }
+  const double lambda = std::min(4.0, std::max(0.05, 0.4 * margin / (rmed + (double)kTol)));
+  // |c_a| <= h_a + M + r_max for a list entry (plus the FP16 rounding),
+  // |p_a| <= h_a + M in the safety box, ell^2 below 4 p2
+  const double R = rmax + (double)kTol;
+  double c2 = 0.0, p2 = 0.0, cmax = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double ca = hd[a] + M + rmax;
+    c2 += ca * ca;
+    p2 += (hd[a] + M) * (hd[a] + M);
+    cmax = std::max(cmax, ca);
+  }
+  const double kmax = 1.01 * (c2 + (1.0 + lambda) * (R + 0.01 * cmax) * (R + 0.01 * cmax));
+  const double vmax = std::max(2.0 * cmax, kmax);
+  int e2 = 0;
+  (void)std::frexp(16384.0 / vmax, &e2);  // sigma = 2^(e2 - 1) <= 16384 / vmax
+  const double sigma = std::ldexp(1.0, e2 - 1);
+  g.reach_lambda = lambda;
+  g.reach_sigma = (float)sigma;
+  g.reach_lam1 = (float)(sigma * (1.0 + 1.0 / lambda));
+  // the FP32 test's terms are below sigma B; six roundings stay below 4e-7 sigma B
+  const double B = 2.0 * kmax + 2.0 * p2 + 4.0 * (1.0 + 1.0 / lambda) * p2;
+  g.reach_eps = (float)(1e-6 * sigma * B);
   return 0;
+}
+
+// binary16 helpers for the reach entries: exact integer and power-of-two
+// steps only (frexp, ldexp, rint), so the host and the device builder agree
+// bit for bit.  Finite |x| < 65504.
+__host__ __device__ inline double half_value(uint16_t h) {
+  const int e = (h >> 10) & 31;
+  const double m = (double)(h & 1023);
+  const double v = e ? std::ldexp(1024.0 + m, e - 25) : std::ldexp(m, -24);
+  return (h & 0x8000) ? -v : v;
+}
+
+// nearest binary16 (ties to even)
+__host__ __device__ inline uint16_t half_rn(double x) {
+  const uint16_t sign = std::signbit(x) ? 0x8000 : 0;
+  const double a = std::fabs(x);
+  if (a == 0.0) return sign;
+  int e = 0;
+  (void)std::frexp(a, &e);            // a in [2^(e-1), 2^e)
+  int eh = std::max(e - 1, -14);      // binade of the result (subnormals: -14)
+  double m = std::rint(std::ldexp(a, 10 - eh));  // a / ulp, rounded
+  if (m >= 2048.0) {                  // rounded up into the next binade
+    m *= 0.5;
+    eh += 1;
+  }
+  const uint32_t mi = (uint32_t)m;
+  if (mi < 1024u) return (uint16_t)(sign | mi);  // subnormal
+  return (uint16_t)(sign | ((uint32_t)(eh + 15) << 10) | (mi - 1024u));
+}
+
+// largest binary16 <= x
+__host__ __device__ inline uint16_t half_rd(double x) {
+  uint16_t h = half_rn(x);
+  if (half_value(h) > x) {
+    if (h == 0) h = 0x8001;                 // +0 -> -2^-24
+    else if (h & 0x8000) h = (uint16_t)(h + 1);  // more negative
+    else h = (uint16_t)(h - 1);
+  }
+  return h;
+}
+
+// The reach entry of a sphere (centre c relative to the cell corner, radius
+// r) for the grid's lambda and sigma: (h_x, h_y, h_z, k) as binary16 bits in
+// two 32-bit words (h_x | h_y << 16, h_z | k << 16).
+__host__ __device__ inline uint2 make_reach_entry(double cx, double cy, double cz, double r,
+                                                  double lambda, double sigma) {
+#pragma clang fp contract(off)
+  const double c[3] = {cx, cy, cz};
+  uint16_t h[3];
+  double ct2 = 0.0, d2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    h[a] = half_rn(-2.0 * sigma * c[a]);
+    const double ct = half_value(h[a]) / (-2.0 * sigma);  // the centre h encodes (exact)
+    ct2 += ct * ct;
+    d2 += (c[a] - ct) * (c[a] - ct);
+  }
+  const double Rp = r + (double)kTol + std::sqrt(d2) * (1.0 + 1e-9);
+  const double q = (1.0 + lambda) * Rp * Rp;
+  // FP64 round-off of k before its downward rounding: below 1e-12 of the terms
+  const double k = sigma * (ct2 - q) - 1e-12 * sigma * (ct2 + q);
+  const uint16_t kh = half_rd(k);
+  return make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)kh << 16));
+}
+
+// The two records of a list entry (centre relative to the cell's lower
+// corner, radius, structure id, sphere index), shared by the host and the
+// device builder.
+__host__ __device__ inline void make_records(double cx, double cy, double cz, double r, double sid,
+                                             int32_t idx, float4& a, float4& b) {
+#pragma clang fp contract(off)
+  const double tol = (double)kTol;
+  float idx_bits;
+  __builtin_memcpy(&idx_bits, &idx, sizeof(idx_bits));
+  a = make_float4((float)cx, (float)cy, (float)cz, (float)((r + tol) * (r + tol)));
+  b = make_float4((float)(r * r), (float)((r - tol) * (r - tol)), (float)sid, idx_bits);
 }
 
 // Returns 0, or -1 with a message in err.
@@ -139,14 +252,6 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
         any = any && lo[a] <= hi[a];
       }
       if (!any) continue;
-      const double ri = r[i];
-      const int32_t idx = (int32_t)i;
-      float idx_bits;
-      std::memcpy(&idx_bits, &idx, sizeof(idx_bits));
-      const float rout2 = (float)((ri + (double)kTol) * (ri + (double)kTol));
-      const float4 bv = make_float4((float)(ri * ri),
-                                    (float)((ri - (double)kTol) * (ri - (double)kTol)),
-                                    (float)sd[i], idx_bits);
       for (int z = lo[2]; z <= hi[2]; ++z)
         for (int y = lo[1]; y <= hi[1]; ++y)
           for (int x = lo[0]; x <= hi[0]; ++x) {
@@ -155,16 +260,39 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
               cnt[c + 1]++;
             } else {
               const int64_t k = fill[c]++;
-              out.cand[2 * k] = make_float4((float)(cx[i] - cell_lo(0, x)),
-                                            (float)(cy[i] - cell_lo(1, y)),
-                                            (float)(cz[i] - cell_lo(2, z)), rout2);
-              out.cand[2 * k + 1] = bv;
+              make_records(cx[i] - cell_lo(0, x), cy[i] - cell_lo(1, y), cz[i] - cell_lo(2, z),
+                           r[i], sd[i], (int32_t)i, out.cand[2 * k], out.cand[2 * k + 1]);
             }
           }
     }
   }
   out.start.resize(ncells + 1);
   for (int64_t c = 0; c <= ncells; ++c) out.start[c] = (int32_t)cnt[c];
+  {  // reach entries: each list from an even index, padding entries at the end
+    out.reach_start.resize(ncells);
+    int64_t rs = 0;
+    for (int64_t c = 0; c < ncells; ++c) {
+      out.reach_start[c] = (int32_t)rs;
+      rs += (cnt[c + 1] - cnt[c] + 1) & ~(int64_t)1;
+    }
+    if ((rs + kReachPad) * 8 >= ((int64_t)1 << 31)) {
+      err = "reach entries exceed 2 GiB; lower cell_size or margin";
+      return -1;
+    }
+    out.reach.assign((size_t)(rs + kReachPad), make_uint2(0u, 0u));
+    for (int z = 0; z < g.nloc[2]; ++z)
+      for (int y = 0; y < g.nloc[1]; ++y)
+        for (int x = 0; x < g.nloc[0]; ++x) {
+          const int64_t c = ((int64_t)z * g.nloc[1] + y) * g.nloc[0] + x;
+          for (int64_t e = cnt[c]; e < cnt[c + 1]; ++e) {
+            int32_t i;
+            std::memcpy(&i, &out.cand[2 * e + 1].w, sizeof(i));
+            out.reach[out.reach_start[c] + (e - cnt[c])] =
+                make_reach_entry(cx[i] - cell_lo(0, x), cy[i] - cell_lo(1, y),
+                                 cz[i] - cell_lo(2, z), r[i], g.reach_lambda, g.reach_sigma);
+          }
+        }
+  }
   {  // sort ranks of the in-box cells: R^3-cell regions, then list length
     const int R = NZ_RANK_REGION;
     const int nx = g.ncell[0], ny = g.ncell[1], nz = g.ncell[2];

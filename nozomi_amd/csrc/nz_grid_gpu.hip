@@ -111,11 +111,20 @@ __global__ void grid_fill_kernel(const GridGeo G, int32_t* cursor, int32_t* ent)
       }
 }
 
+// Padded length of each list in the reach array (even: lists start at even
+// entries); the entry after the last cell is 0 for the scan.
+__global__ void grid_reach_count_kernel(const int32_t* start, int64_t ncells, int32_t* rcnt) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncells) return;
+  rcnt[c] = c < ncells ? ((start[c + 1] - start[c] + 1) & ~1) : 0;
+}
+
 // One thread per cell: sort the list by sphere index, then write the records
-// cand[2k] = (cx - lo_x, cy - lo_y, cz - lo_z, (r+tol)^2),
-// cand[2k+1] = (r^2, (r-tol)^2, sid, sphere index bits).
+// and the reach entries (make_records, make_reach_entry: the host builder's
+// arithmetic).
 __global__ void grid_records_kernel(const GridGeo G, const int32_t* start, int64_t ncells,
-                                    int32_t* ent, float4* cand) {
+                                    int32_t* ent, float4* cand, const int32_t* reach_start,
+                                    uint2* reach, double lambda, double sigma) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const int32_t b = start[c], e = start[c + 1];
@@ -132,17 +141,13 @@ __global__ void grid_records_kernel(const GridGeo G, const int32_t* start, int64
   const int y = (int)((c / G.nloc[0]) % G.nloc[1]);
   const int z = (int)(c / ((int64_t)G.nloc[0] * G.nloc[1]));
   const double lx = geo_cell_lo(G, 0, x), ly = geo_cell_lo(G, 1, y), lz = geo_cell_lo(G, 2, z);
-  const double tol = (double)kTol;
+  const int32_t rs = reach_start[c];
   for (int32_t k = b; k < e; ++k) {
     const int32_t i = ent[k];
-    const double ri = G.sp[3 * G.S + i];
-    float idx_bits;
-    __builtin_memcpy(&idx_bits, &i, sizeof(idx_bits));
-    cand[2 * (int64_t)k] = make_float4((float)(G.sp[i] - lx), (float)(G.sp[G.S + i] - ly),
-                                       (float)(G.sp[2 * G.S + i] - lz),
-                                       (float)((ri + tol) * (ri + tol)));
-    cand[2 * (int64_t)k + 1] = make_float4((float)(ri * ri), (float)((ri - tol) * (ri - tol)),
-                                           (float)G.sp[4 * G.S + i], idx_bits);
+    const double cx = G.sp[i] - lx, cy = G.sp[G.S + i] - ly, cz = G.sp[2 * G.S + i] - lz;
+    make_records(cx, cy, cz, G.sp[3 * G.S + i], G.sp[4 * G.S + i], i, cand[2 * (int64_t)k],
+                 cand[2 * (int64_t)k + 1]);
+    reach[rs + (k - b)] = make_reach_entry(cx, cy, cz, G.sp[3 * G.S + i], lambda, sigma);
   }
 }
 
@@ -284,8 +289,26 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
     hipLaunchKernelGGL(grid_fill_kernel, dim3(blocks256(S)), dim3(256), 0, st, G, cursor, ent);
     hipLaunchKernelGGL(grid_spheres_kernel, dim3(blocks256(S)), dim3(256), 0, st, G, sph, sid);
   }
+  // reach starts: scan of the padded list lengths (the partial buffer of the
+  // first scan is free again: same stream)
+  int32_t* reach_start = nullptr;
+  if (int rc = dev_alloc(s, (size_t)ncells + 1, &reach_start)) return rc;
+  hipLaunchKernelGGL(grid_reach_count_kernel, dim3(blocks256(ncells + 1)), dim3(256), 0, st,
+                     (const int32_t*)start, ncells, reach_start);
+  if (int rc = exclusive_scan_i32(reach_start, ncells + 1, partial, st)) return rc;
+  int32_t rtotal = 0;
+  NZ_HIP(hipMemcpyAsync(&rtotal, reach_start + ncells, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  NZ_HIP(hipStreamSynchronize(st));
+  if (((int64_t)rtotal + kReachPad) * 8 >= ((int64_t)1 << 31))
+    return fail(-1, "nz_substrate_create: reach entries exceed 2 GiB; lower cell_size or margin");
+  uint2* reach = nullptr;
+  if (int rc = dev_alloc(s, (size_t)rtotal + kReachPad, &reach)) return rc;
+  const int64_t rwords = 2 * ((int64_t)rtotal + kReachPad);
+  hipLaunchKernelGGL(grid_zero_kernel, dim3(blocks256(rwords)), dim3(256), 0, st, (int32_t*)reach,
+                     rwords);
   hipLaunchKernelGGL(grid_records_kernel, dim3(blocks256(ncells)), dim3(256), 0, st, G,
-                     (const int32_t*)start, ncells, ent, cand);
+                     (const int32_t*)start, ncells, ent, cand, (const int32_t*)reach_start, reach,
+                     g.reach_lambda, (double)g.reach_sigma);
   // 5. sort ranks
   hipLaunchKernelGGL(grid_rank_kernel, dim3(blocks256(nbox)), dim3(256), 0, st, G,
                      (const int32_t*)start, rank);
@@ -309,6 +332,9 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
   g.cell_rank = rank;
   g.cand = cand;
   g.cand_bytes = (int32_t)(sizeof(float4) * 2 * ((size_t)total + kCandPad));
+  g.reach = reach;
+  g.reach_start = reach_start;
+  g.reach_bytes = (int32_t)(sizeof(uint2) * ((size_t)rtotal + kReachPad));
   g.sph = sph;
   g.sph_sid = sid;
   g.D = Dd;

@@ -26,6 +26,9 @@ constexpr int32_t kSeedAttempts = 1 << 20;
 // Zero records after the last candidate list (the scans load up to 15
 // records past a list's end).
 constexpr int kCandPad = 16;
+// padding entries after the last reach list (the first pass loads up to 15
+// entries past a list)
+constexpr int kReachPad = 16;
 // Most candidate entries of a substrate: the record array (32 bytes an entry,
 // padding included) stays below 2 GiB, the range of the scan's buffer loads
 // and of the int32 byte size it is described by.
@@ -48,6 +51,25 @@ struct Grid {
   double hd[3];        // cell edge (FP64)
   double Ld[3];        // box edge (FP64)
   float margin;        // safety margin of the lists
+  // The reach test of the scan's first pass, per list entry (R = r + tol,
+  // c the centre relative to the cell corner, ell the segment length, p the
+  // position relative to the cell corner): a segment that touches the
+  // R-ball has |p - c| <= R + ell, hence for any lambda > 0
+  //   |p - c|^2 <= (1 + lambda) R^2 + (1 + 1/lambda) ell^2,
+  // evaluated as p.h + k <= sigma ((1 + 1/lambda) ell^2 - |p|^2) + eps from
+  // an 8-byte entry of FP16 values (h_x, h_y, h_z, k): h = -2 sigma c~ (c~
+  // the centre h encodes), k <= sigma (|c~|^2 - (1 + lambda) R'^2) with R'
+  // = R + |c - c~|.  sigma is a power of two that keeps the values in FP16
+  // range; eps bounds the FP32 round-off of the test.
+  float reach_sigma;
+  float reach_lam1;    // sigma (1 + 1/lambda)
+  float reach_eps;
+  double reach_lambda;
+  // entries of cell c's list from reach[reach_start[c]] (an even index:
+  // two entries per 16-byte load), padding entries at the end
+  const uint2* reach;
+  const int32_t* reach_start;  // [nloc^3]
+  int32_t reach_bytes;         // size of reach in bytes (< 2^31)
   int32_t nstruct;     // number of structures; the box has id nstruct
   int64_t nspheres;
   // CSR candidate lists over the nloc^3 local cells; entry e is the 32-byte

@@ -84,6 +84,20 @@ int nz_substrate_export_grid(nz_substrate_t sub, int32_t* cell_start, int32_t* c
   return 0;
 }
 
+int nz_substrate_export_reach(nz_substrate_t sub, int64_t* nentries, int32_t* reach_start,
+                              uint32_t* reach) {
+  if (!sub) return fail(-1, "nz_substrate_export_reach: NULL substrate");
+  const Substrate* s = (const Substrate*)sub;
+  const Grid& g = s->g;
+  const size_t ncells = (size_t)g.nloc[0] * g.nloc[1] * g.nloc[2];
+  const size_t n = (size_t)g.reach_bytes / sizeof(uint2);
+  if (nentries) *nentries = (int64_t)n;
+  if (reach_start)
+    NZ_HIP(hipMemcpy(reach_start, g.reach_start, sizeof(int32_t) * ncells, hipMemcpyDeviceToHost));
+  if (reach) NZ_HIP(hipMemcpy(reach, g.reach, sizeof(uint2) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int nz_substrate_destroy(nz_substrate_t sub) {
   if (!sub) return 0;
   Substrate* s = (Substrate*)sub;

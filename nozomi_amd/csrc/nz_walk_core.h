@@ -138,10 +138,9 @@ struct ScanAcc {
 struct ScanIn {
   float p[3], j[3];
   float a4, inv_a, sid;  // 4 |j|^2, 1 / |j|^2, structure id assumed for the scan
-  // reach bound (scan_list): with l the sub-step's segment length plus slack,
-  // a sphere whose (r+tol)-ball the segment cannot touch has
-  // dd > (r+tol)^2 reach1 + reach0
-  float reach1, reach0;
+  // right-hand side of the reach test (scan_list): with ell the sub-step's
+  // segment length plus slack, sigma ((1 + 1/lambda) ell^2 - |p|^2) + eps
+  float rhs;
 };
 
 // Accumulators of the FP32 candidate loop.  A decision and the list position
@@ -369,6 +368,20 @@ struct RecLoader {
   __device__ float4 a(int e) const { return ld(base + e * 32, 0); }
   __device__ float4 b(int e) const { return ld(base + e * 32 + 16, 0); }
 };
+// Loads of the reach entries of one list (from entry rs, an even index).
+struct ReachLoader {
+  __amdgpu_buffer_rsrc_t r;
+  int base;  // byte offset of the list's first entry (16-byte aligned)
+  __device__ ReachLoader(const Grid& g, int rs)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void*)g.reach, (short)0, g.reach_bytes, 0x00020000)),
+        base(rs * 8) {}
+  // entries e and e + 1 (e even, the same on every active lane: the loop
+  // counter of the scan, taken from the first lane so the offset is scalar)
+  __device__ nz_u32x4 ld2(int e) const {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, base, __builtin_amdgcn_readfirstlane(e * 8),
+                                                 0);
+  }
+};
 #else
 NZ_HD inline void sched_fence() {}
 struct RecLoader {
@@ -377,6 +390,14 @@ struct RecLoader {
   float4 a_u(int e) const { return p[2 * e]; }
   float4 a(int e) const { return p[2 * e]; }
   float4 b(int e) const { return p[2 * e + 1]; }
+};
+struct Words4 {
+  uint32_t x, y, z, w;
+};
+struct ReachLoader {
+  const uint2* p;
+  ReachLoader(const Grid& g, int rs) : p(g.reach + rs) {}
+  Words4 ld2(int e) const { return Words4{p[e].x, p[e].y, p[e + 1].x, p[e + 1].y}; }
 };
 #endif
 
@@ -399,22 +420,46 @@ NZ_HD inline void scan_all(const float4* rec, int n, const ScanIn& in, ScanKeys&
   if (e < n) visit(in, A0, B0, (uint32_t)(e + 1), k);
 }
 
-// Can the sub-step's segment touch the candidate's (r+tol)-ball?  A sphere it
-// cannot touch neither contains the spin (dd > (r+tol)^2 > r^2), nor lies in
-// the inside test's guard band, nor can be entered before jr: it cannot
-// change the keys.  The bound avoids r: 2R <= R^2 + 1 gives (R + l)^2 <=
-// R^2 (1 + l) + l + l^2, so dd above that is out of reach.
-NZ_HD inline bool reachable(const ScanIn& in, const float4 A) {
-  const float dx = in.p[0] - A.x, dy = in.p[1] - A.y, dz = in.p[2] - A.z;
-  const float dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-  return dd <= fmaf(A.w, in.reach1, in.reach0);
+// Can the sub-step's segment (start p, length at most ell) touch the
+// candidate's R-ball (R = r + tol)?  Only if |p - c| <= R + ell; the reach
+// entry (Grid::reach) turns the weaker |p - c|^2 <= (1 + lambda) R^2 +
+// (1 + 1/lambda) ell^2 into three FMAs on its FP16 values, with every
+// rounding on the side of passing.  A sphere that fails neither contains the
+// spin nor can be entered before jr: it cannot change the exit key, the entry
+// key within jr or the structure id.  w0 = h_x | h_y << 16, w1 = h_z | k << 16.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef _Float16 nz_h2 __attribute__((ext_vector_type(2)));
+NZ_HD inline bool reach_entry(const ScanIn& in, uint32_t w0, uint32_t w1) {
+  const nz_h2 a = __builtin_bit_cast(nz_h2, w0);
+  const nz_h2 b = __builtin_bit_cast(nz_h2, w1);
+  // FP16 operands of FP32 FMAs: v_fma_mix_f32, no conversions
+  float t = fmaf(in.p[0], (float)a.x, (float)b.y);
+  t = fmaf(in.p[1], (float)a.y, t);
+  t = fmaf(in.p[2], (float)b.x, t);
+  return t <= in.rhs;
 }
+#else
+inline float half_to_float(uint32_t h) {  // exact
+  const int e = (int)((h >> 10) & 31u);
+  const float m = (float)(h & 1023u);
+  const float v = e ? std::ldexp(1024.0f + m, e - 25) : std::ldexp(m, -24);
+  return (h & 0x8000u) ? -v : v;
+}
+inline bool reach_entry(const ScanIn& in, uint32_t w0, uint32_t w1) {
+  float t = fmaf(in.p[0], half_to_float(w0 & 0xffffu), half_to_float(w1 >> 16));
+  t = fmaf(in.p[1], half_to_float(w0 >> 16), t);
+  t = fmaf(in.p[2], half_to_float(w1 & 0xffffu), t);
+  return t <= in.rhs;
+}
+#endif
 
-// Reach bits (bit i: candidate A_i) of four records.
-NZ_HD inline uint32_t reach4(const ScanIn& in, const float4 A0, const float4 A1, const float4 A2,
-                             const float4 A3) {
-  return (uint32_t)reachable(in, A0) | ((uint32_t)reachable(in, A1) << 1) |
-         ((uint32_t)reachable(in, A2) << 2) | ((uint32_t)reachable(in, A3) << 3);
+// Reach bits (bit i: entry i) of eight entries, two per 16-byte word group.
+template <class W>
+NZ_HD inline uint32_t reach8(const ScanIn& in, const W& a, const W& b, const W& c, const W& d) {
+  return (uint32_t)reach_entry(in, a.x, a.y) | ((uint32_t)reach_entry(in, a.z, a.w) << 1) |
+         ((uint32_t)reach_entry(in, b.x, b.y) << 2) | ((uint32_t)reach_entry(in, b.z, b.w) << 3) |
+         ((uint32_t)reach_entry(in, c.x, c.y) << 4) | ((uint32_t)reach_entry(in, c.z, c.w) << 5) |
+         ((uint32_t)reach_entry(in, d.x, d.y) << 6) | ((uint32_t)reach_entry(in, d.z, d.w) << 7);
 }
 
 // Lowest set position of the mask, cleared; `none` when it is empty.
@@ -425,42 +470,46 @@ NZ_HD inline int pop_flag(uint64_t& m, int none) {
   return q;
 }
 
-// The candidate scan over the list [cs, ce).  A reach pass over every
-// candidate (its first float4 only) marks the spheres the segment can touch
-// as bits of a 64-bit mask (the loop position is the same on every lane);
-// only those get the full visit, a few per list against ~15 candidates.
-// Against visiting every candidate the exit key and the structure id are
-// equal; the entry key can only differ by entries beyond jr (the skipped
-// obstacles'), which cannot decide the sub-step, and the guard count by
-// out-of-reach spheres, whose FP64 re-solve could not change it either.
-// Lists longer than 64 take the full visit for every candidate.
-NZ_HD inline void scan_list(const Grid& g, int cs, int ce, const ScanIn& in, ScanKeys& k) {
+// The candidate scan over the list [cs, ce) (reach entries from rs).  A
+// reach pass over every candidate (its 8-byte reach entry) marks the spheres
+// the segment can touch as bits of a 64-bit mask (the loop position is the
+// same on every lane); only those get the full visit, a few per list against
+// ~20 candidates.  Against visiting every candidate the exit key and the
+// structure id are equal; the entry key can only differ by entries beyond jr
+// (the skipped obstacles'), which cannot decide the sub-step, and the guard
+// count by out-of-reach spheres, whose FP64 re-solve could not change it
+// either.  Lists longer than 64 take the full visit for every candidate.
+NZ_HD inline void scan_list(const Grid& g, int cs, int ce, int rs, const ScanIn& in,
+                            ScanKeys& k) {
   const int n = ce - cs;
   if (n > 64) {
     scan_all(g.cand + 2 * (int64_t)cs, n, in, k);
     return;
   }
   const RecLoader rl(g, cs);
-  // eight candidates per trip in two groups of four, each group's records
-  // loaded while the other group is tested (the two register sets alternate
-  // roles, so nothing is copied).  Positions past the list (loads stay below
-  // n + 16) are tested too and masked off after the loop.
+  const ReachLoader ql(g, rs);
+  // eight entries (four 16-byte loads) per group, two groups alternating:
+  // one group's loads are in flight while the other is tested, and the
+  // register sets swap roles, so nothing is copied.  Positions past the list
+  // (loads stay below n + 16) are tested too and masked off after the loop.
   uint64_t m = 0u;
-  float4 A0 = rl.a_u(0), A1 = rl.a_u(1), A2 = rl.a_u(2), A3 = rl.a_u(3);
+  auto A0 = ql.ld2(0), A1 = ql.ld2(2), A2 = ql.ld2(4), A3 = ql.ld2(6);
   int e = 0;
-  do {
-    const float4 N0 = rl.a_u(e + 4), N1 = rl.a_u(e + 5), N2 = rl.a_u(e + 6), N3 = rl.a_u(e + 7);
+  while (true) {
+    const auto N0 = ql.ld2(e + 8), N1 = ql.ld2(e + 10), N2 = ql.ld2(e + 12), N3 = ql.ld2(e + 14);
     sched_fence();
-    uint32_t b = reach4(in, A0, A1, A2, A3);
-    A0 = rl.a_u(e + 8);
-    A1 = rl.a_u(e + 9);
-    A2 = rl.a_u(e + 10);
-    A3 = rl.a_u(e + 11);
-    sched_fence();
-    b |= reach4(in, N0, N1, N2, N3) << 4;
-    m |= (uint64_t)b << e;
+    m |= (uint64_t)reach8(in, A0, A1, A2, A3) << e;
     e += 8;
-  } while (e < n);
+    if (e >= n) break;
+    A0 = ql.ld2(e + 8);
+    A1 = ql.ld2(e + 10);
+    A2 = ql.ld2(e + 12);
+    A3 = ql.ld2(e + 14);
+    sched_fence();
+    m |= (uint64_t)reach8(in, N0, N1, N2, N3) << e;
+    e += 8;
+    if (e >= n) break;
+  }
   m &= n >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << n) - 1u);
   // the marked records, each loaded while the previous one is visited; the
   // position n (none left) loads the record after the list, not visited
@@ -627,8 +676,29 @@ struct StepState {
   double j[3];  // current jump vector
   int niter;    // sub-steps done in this step
   int cs, ce;   // candidate list of the lane's cell
+  int rs;       // its first reach entry
   int pending;  // draw index of the jump still to draw before the next sub-step (-1: none)
 };
+
+// The FP32 scan's inputs for position rel (relative to the lane's cell
+// corner), jump j, remaining fraction jr and assumed structure id sid.
+NZ_HD inline void scan_in(const Grid& g, const double rel[3], const double j[3], double jr,
+                          float sid, ScanIn& in) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    in.p[a] = (float)rel[a];
+    in.j[a] = (float)j[a];
+  }
+  const float a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
+  in.a4 = 4.0f * a;
+  in.inv_a = 1.0f / a;
+  in.sid = sid;
+  // segment length with slack (relative 1e-3, absolute 1e-4 um) for the
+  // FP32 round-off of its length
+  const float ell = sqrtf(a) * (float)jr * 1.001f + 1e-4f;
+  const float pp = fmaf(in.p[2], in.p[2], fmaf(in.p[1], in.p[1], in.p[0] * in.p[0]));
+  in.rhs = fmaf(g.reach_lam1, ell * ell, g.reach_eps) - g.reach_sigma * pp;
+}
 
 // Start a time step.  The first jump (draw 0) is drawn with the structure
 // id of the previous sub-step; substep() redraws it if the structure pass
@@ -644,6 +714,7 @@ NZ_HD inline void step_begin(const Grid& g, const Lane& s, StepState& q, double 
   const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
   q.cs = g.cell_start[cid];
   q.ce = g.cell_start[cid + 1];
+  q.rs = g.reach_start[cid];
 }
 
 // Draw the pending jump: draw 0 at the start of a step, draw k after the
@@ -684,6 +755,7 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
         const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
         q.cs = g.cell_start[cid];
         q.ce = g.cell_start[cid + 1];
+        q.rs = g.reach_start[cid];
         if (!in_safety(g, rel, q.j, q.jr))
           exact_mode = in_neighbourhood(g, rel, q.j, q.jr, s.l) ? kExactNeighbours : kExactAll;
       } else {
@@ -694,23 +766,10 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
     if (exact_mode < 0 && q.ce - q.cs > kMaxListLen) exact_mode = kExactList;  // dense cell
     if (exact_mode < 0) {
       ScanIn in;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        in.p[a] = (float)rel[a];
-        in.j[a] = (float)q.j[a];
-      }
-      const float a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
-      in.a4 = 4.0f * a;
-      in.inv_a = 1.0f / a;
-      in.sid = s.sid;
-      // segment length with slack (relative 1e-3, absolute 1e-4 um) for the
-      // FP32 round-off of the reach bound
-      const float ell = sqrtf(a) * (float)q.jr * 1.001f + 1e-4f;
-      in.reach1 = 1.0f + ell;
-      in.reach0 = ell + ell * ell;
+      scan_in(g, rel, q.j, q.jr, s.sid, in);
       ScanKeys k;
       keys_reset(g, k);
-      scan_list(g, q.cs, q.ce, in, k);
+      scan_list(g, q.cs, q.ce, q.rs, in, k);
       if (k.gcnt > 1) {
         exact_mode = kExactList;
       } else {

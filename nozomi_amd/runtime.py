@@ -158,6 +158,21 @@ class DeviceSubstrate:
               "nz_substrate_export_grid")
         return start, rank, cand
 
+    def reach(self):
+        """The device-built reach entries (nz_substrate_export_reach):
+        reach_start int32 [nloc^3], reach uint32 [nentries, 2]."""
+        inf = self.info()
+        nloc = [n + 2 for n in inf["ncell"]]
+        n = ctypes.c_int64(0)
+        check(lib().nz_substrate_export_reach(self.handle, ctypes.byref(n), None, None),
+              "nz_substrate_export_reach")
+        rs = np.empty(nloc[0] * nloc[1] * nloc[2], np.int32)
+        reach = np.empty((n.value, 2), np.uint32)
+        check(lib().nz_substrate_export_reach(self.handle, None, rs.ctypes.data_as(_c_vp),
+                                              reach.ctypes.data_as(_c_vp)),
+              "nz_substrate_export_reach")
+        return rs, reach
+
     def close(self):
         if getattr(self, "handle", None):
             lib().nz_substrate_destroy(self.handle)

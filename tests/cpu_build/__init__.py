@@ -40,6 +40,9 @@ def lib():
         L.nzcpu_info.argtypes = [_vp, _vp]
         L.nzcpu_grid.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.nzcpu_scan_compare.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+        L.nzcpu_reach_check.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
+        L.nzcpu_reach.argtypes = [_vp, _vp, _vp, _vp]
+        L.nzcpu_reach_params.argtypes = [_vp, _vp]
         L.nzcpu_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                     ctypes.c_uint32, ctypes.c_int, _vp]
         _lib = L
@@ -77,6 +80,35 @@ class CpuWalker:
         lib().nzcpu_scan_compare(self.h, n, pos.ctypes.data, jump.ctypes.data, jr.ctypes.data,
                                  sid.ctypes.data, out.ctypes.data)
         return out
+
+    def reach_check(self, pos, jump, jr):
+        """Per probe: int32 [n, 3] = (candidates the segment touches that the
+        FP32 reach test rejects, candidates it touches (FP64), candidates
+        the reach test passes)."""
+        n = pos.shape[1]
+        pos = np.ascontiguousarray(pos, np.float64)
+        jump = np.ascontiguousarray(jump, np.float64)
+        jr = np.ascontiguousarray(jr, np.float64)
+        out = np.zeros((n, 3), np.int32)
+        lib().nzcpu_reach_check(self.h, n, pos.ctypes.data, jump.ctypes.data, jr.ctypes.data,
+                                out.ctypes.data)
+        return out
+
+    def reach(self):
+        """The host builder's (reach_start int32 [nloc^3], reach uint32
+        [entries, 2])."""
+        sizes = np.zeros(2, np.int64)
+        lib().nzcpu_reach(self.h, None, None, sizes.ctypes.data)
+        rs = np.empty(sizes[1], np.int32)
+        reach = np.empty((sizes[0], 2), np.uint32)
+        lib().nzcpu_reach(self.h, rs.ctypes.data, reach.ctypes.data, sizes.ctypes.data)
+        return rs, reach
+
+    def reach_params(self):
+        """(lambda, sigma, sigma (1 + 1/lambda), eps) of the reach test."""
+        out = np.zeros(4, np.float64)
+        lib().nzcpu_reach_params(self.h, out.ctypes.data)
+        return tuple(out)
 
     def grid(self):
         """The host builder's (cell_start, cell_rank, cand [2 records, 4]), padding

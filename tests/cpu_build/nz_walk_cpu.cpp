@@ -70,21 +70,11 @@ void nzcpu_scan_compare(void* h, int n, const double* pos, const double* jump, c
     double rel[3];
     lane_rel(g, s, rel);
     ScanIn in;
-    for (int a = 0; a < 3; ++a) {
-      in.p[a] = (float)rel[a];
-      in.j[a] = (float)q.j[a];
-    }
-    const float a2 = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
-    in.a4 = 4.0f * a2;
-    in.inv_a = 1.0f / a2;
-    in.sid = sid[i];
-    const float ell = sqrtf(a2) * (float)q.jr * 1.001f + 1e-4f;
-    in.reach1 = 1.0f + ell;
-    in.reach0 = ell + ell * ell;
+    scan_in(g, rel, q.j, q.jr, sid[i], in);
     ScanKeys k2, k1;
     keys_reset(g, k2);
     keys_reset(g, k1);
-    scan_list(g, cs, ce, in, k2);
+    scan_list(g, cs, ce, g.reach_start[cid], in, k2);
     scan_all(g.cand + 2 * (int64_t)cs, ce - cs, in, k1);
     out[4 * i + 0] = k2.kin != k1.kin;
     const float lim = (float)q.jr * 1.0001f + 1e-6f;
@@ -94,6 +84,75 @@ void nzcpu_scan_compare(void* h, int n, const double* pos, const double* jump, c
     out[4 * i + 2] = k2.sid_new != k1.sid_new;
     out[4 * i + 3] = (k2.gcnt > k1.gcnt) || (k1.gcnt == 1 && k2.gcnt == 1 && k1.gpos != k2.gpos);
   }
+}
+
+// The reach bound against the FP64 distance from the segment p -> p + jr j
+// to each candidate's centre.  out[3 i] = candidates whose R-ball (R = r +
+// tol) the segment touches but the FP32 reach test rejects (must be 0),
+// out[3 i + 1] = candidates the segment touches, out[3 i + 2] = candidates
+// the reach test passes.
+void nzcpu_reach_check(void* h, int n, const double* pos, const double* jump, const double* jr,
+                       int32_t* out) {
+  const Grid& g = ((CpuWalker*)h)->hg.g;
+  for (int i = 0; i < n; ++i) {
+    Lane s;
+    const float p[3] = {(float)pos[i], (float)pos[n + i], (float)pos[2 * n + i]};
+    lane_init(g, p, s);
+    double j[3];
+    for (int a = 0; a < 3; ++a) j[a] = jump[a * n + i];
+    const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
+    double rel[3];
+    lane_rel(g, s, rel);
+    ScanIn in;
+    scan_in(g, rel, j, jr[i], 0.0f, in);
+    int32_t miss = 0, touch = 0, pass = 0;
+    for (int e = g.cell_start[cid]; e < g.cell_start[cid + 1]; ++e) {
+      const int k = cand_sphere(g, e);
+      const double4 c = g.sph[k];
+      // closest point of the segment to the centre
+      double d[3], jj = 0.0, dj = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        d[a] = c.x * (a == 0) + c.y * (a == 1) + c.z * (a == 2) - s.x[a];
+        jj += j[a] * j[a] * jr[i] * jr[i];
+        dj += d[a] * j[a] * jr[i];
+      }
+      const double t = jj > 0 ? std::min(1.0, std::max(0.0, dj / jj)) : 0.0;
+      double dist2 = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        const double v = d[a] - t * j[a] * jr[i];
+        dist2 += v * v;
+      }
+      const double R = c.w + (double)kTol;
+      const bool touches = dist2 <= R * R;
+      const uint2 w = g.reach[g.reach_start[cid] + (e - g.cell_start[cid])];
+      const bool passes = reach_entry(in, w.x, w.y);
+      touch += touches;
+      pass += passes;
+      miss += touches && !passes;
+    }
+    out[3 * i] = miss;
+    out[3 * i + 1] = touch;
+    out[3 * i + 2] = pass;
+  }
+}
+
+// The reach test's parameters: lambda, sigma, sigma (1 + 1/lambda), eps.
+void nzcpu_reach_params(void* h, double* out) {
+  const Grid& g = ((CpuWalker*)h)->hg.g;
+  out[0] = g.reach_lambda;
+  out[1] = g.reach_sigma;
+  out[2] = g.reach_lam1;
+  out[3] = g.reach_eps;
+}
+
+// The host builder's reach entries: sizes[0] = entries (padding included),
+// sizes[1] = cells; reach_start [cells], reach [2 entries] words.
+void nzcpu_reach(void* h, int32_t* reach_start, uint32_t* reach, int64_t* sizes) {
+  const HostGrid& hg = ((CpuWalker*)h)->hg;
+  sizes[0] = (int64_t)hg.reach.size();
+  sizes[1] = (int64_t)hg.reach_start.size();
+  if (reach_start) std::memcpy(reach_start, hg.reach_start.data(), sizeof(int32_t) * sizes[1]);
+  if (reach) std::memcpy(reach, hg.reach.data(), sizeof(uint2) * sizes[0]);
 }
 
 // the walk's normals (host build of nz::normals3), n draws from draw0

@@ -98,6 +98,87 @@ def test_oracle_is_chaotic(c0):
     assert np.mean(err[res[0][1] <= 10] < 1e-7) > 0.99   # collision-poor spins do not
 
 
+def _probes(g, rng, n_u, n_s):
+    """Uniform positions and positions on and near sphere surfaces (inside
+    and outside the tolerance shells), wrapped into the box."""
+    L = g.Lx
+    sp = g.spheres
+    pos_u = rng.uniform(-L / 2, L / 2, size=(3, n_u))
+    k = rng.integers(0, sp.shape[1], n_s)
+    d = rng.normal(size=(3, n_s))
+    d /= np.linalg.norm(d, axis=0)
+    shell = sp[3, k] + rng.choice([-1.0, 1.0], n_s) * 10.0 ** rng.uniform(-7, -3, n_s)
+    pos = np.concatenate([pos_u, sp[:3, k] + d * shell], axis=1)
+    pos = pos - L * np.round(pos / L)
+    return np.clip(pos, -L / 2 + 1e-6, L / 2 - 1e-6)
+
+
+@pytest.mark.parametrize("cell", [1.25, 4.0])
+def test_reach_entries_round_as_numpy_float16(c0, cell):
+    """Every reach entry of the host grid: h = -2 sigma c rounded to the
+    nearest binary16 (numpy's float16 conversion from FP64, ties to even),
+    and k the largest binary16 at or below sigma (|c'|^2 - (1 + lambda)
+    R'^2) (c' the centre h encodes, R' = r + tol + |c - c'|) less the FP64
+    round-off allowance.  Lists start at even entries; the gaps and the
+    padding are zero."""
+    g, _ = c0
+    cw = CpuWalker(g, cell=cell, margin=0.8)
+    lam, sigma, lam1, eps = cw.reach_params()
+    assert sigma == 2.0 ** np.round(np.log2(sigma)) and lam1 == np.float32(sigma * (1 + 1 / lam))
+    start, _, cand = cw.grid()
+    rs, reach = cw.reach()
+    ncells = rs.size
+    lens = np.diff(start)[:ncells]
+    assert np.all(rs % 2 == 0) and np.array_equal(np.diff(rs), (lens[:-1] + 1) & ~1)
+    cell_of = np.repeat(np.arange(ncells), lens)
+    e = np.arange(cell_of.size)
+    pos = rs[cell_of] + (e - start[cell_of])
+    used = np.zeros(reach.shape[0], bool)
+    used[pos] = True
+    assert not reach[~used].any()
+    nloc = round(ncells ** (1 / 3))
+    h = g.Lx / (nloc - 2)
+    lx, ly, lz = cell_of % nloc, (cell_of // nloc) % nloc, cell_of // nloc ** 2
+    lo = np.stack([-g.Lx / 2 + (l - 1) * h for l in (lx, ly, lz)])
+    idx = cand[1::2, 3].view(np.int32)[: cell_of.size]
+    sp = g.spheres
+    c = sp[:3, idx] - lo
+    w = reach[pos]
+    hbits = np.stack([w[:, 0] & 0xFFFF, w[:, 0] >> 16, w[:, 1] & 0xFFFF]).astype(np.uint16)
+    assert np.array_equal(hbits, (-2.0 * sigma * c).astype(np.float16).view(np.uint16))
+    ct = hbits.view(np.float16).astype(np.float64) / (-2.0 * sigma)
+    rp = sp[3, idx] + 1e-4 + np.sqrt(((c - ct) ** 2).sum(0)) * (1 + 1e-9)
+    ct2, q = (ct ** 2).sum(0), (1 + lam) * rp * rp
+    bound = sigma * (ct2 - q) - 1e-12 * sigma * (ct2 + q)
+    k = (w[:, 1] >> 16).astype(np.uint16).view(np.float16)
+    assert np.all(k.astype(np.float64) <= bound)
+    up = np.nextafter(k, np.float16(np.inf)).astype(np.float64)
+    assert np.all(up > bound)
+
+
+@pytest.mark.parametrize("cell", [1.25, 2.0, 4.0])
+def test_reach_bound_keeps_every_touchable_sphere(c0, cell):
+    """The FP32 reach test (three FMAs on the FP16 reach entry) passes every
+    candidate whose R-ball (R = r + tol) the segment touches, judged by the
+    FP64 distance from the segment to the centre: segments from 1e-4 um to
+    the list margin, probes on and near sphere surfaces.  The pass set stays
+    close to the touch set."""
+    g, _ = c0
+    cw = CpuWalker(g, cell=cell, margin=0.8)
+    rng = np.random.default_rng(11)
+    pos = _probes(g, rng, 20000, 40000)
+    n = pos.shape[1]
+    jd = rng.normal(size=(3, n))
+    jd /= np.linalg.norm(jd, axis=0)
+    jump = jd * 10.0 ** rng.uniform(-4, np.log10(0.7), n)
+    jr = 10.0 ** rng.uniform(-4, 0, n)
+    out = cw.reach_check(pos, jump, jr)
+    print(f"touching {out[:, 1].sum()}, passing {out[:, 2].sum()} "
+          f"({out[:, 2].sum() / max(out[:, 1].sum(), 1):.3f}x)")
+    assert out[:, 0].sum() == 0
+    assert out[:, 2].sum() <= 1.6 * out[:, 1].sum()
+
+
 @pytest.mark.parametrize("cell", [1.25, 4.0])
 def test_two_phase_scan_equals_full_scan(c0, cell):
     """The reach pass may only skip spheres that cannot change the sub-step:

@@ -42,6 +42,11 @@ def test_device_grid_equals_host_grid(name):
     assert np.array_equal(rank, hr)
     assert cand.shape == hc.shape
     assert np.array_equal(cand.view(np.uint32), hc.view(np.uint32))
+    # reach entries of the first pass
+    rs, reach = sub.reach()
+    hrs, hreach = CpuWalker(geom, cell=cell, margin=margin).reach()
+    assert np.array_equal(rs, hrs)
+    assert np.array_equal(reach, hreach)
     # summary figures
     lens = np.diff(hs)
     assert inf["ncand_total"] == hs[-1]
