@@ -33,6 +33,7 @@ struct HostGrid {
   std::vector<float4> cand;  // a, b records interleaved
   std::vector<uint2> reach;
   std::vector<int32_t> reach_start;
+  std::vector<int4> cell_tab;
   std::vector<double4> sph;
   std::vector<int32_t> sid;
   std::vector<double> D, T2;
@@ -47,6 +48,7 @@ struct HostGrid {
     g.reach = reach.data();
     g.reach_start = reach_start.data();
     g.reach_bytes = (int32_t)(sizeof(uint2) * reach.size());
+    g.cell_tab = cell_tab.data();
     g.sph = sph.data();
     g.sph_sid = sid.data();
     g.D = D.data();
@@ -280,6 +282,9 @@ inline int build_grid_host(const double* spheres, int64_t S, int32_t nstruct, co
       return -1;
     }
     out.reach.assign((size_t)(rs + kReachPad), make_uint2(0u, 0u));
+    out.cell_tab.resize(ncells);
+    for (int64_t c = 0; c < ncells; ++c)
+      out.cell_tab[c] = make_int4((int32_t)cnt[c], (int32_t)cnt[c + 1], out.reach_start[c], 0);
     for (int z = 0; z < g.nloc[2]; ++z)
       for (int y = 0; y < g.nloc[1]; ++y)
         for (int x = 0; x < g.nloc[0]; ++x) {

@@ -119,6 +119,12 @@ __global__ void grid_reach_count_kernel(const int32_t* start, int64_t ncells, in
   rcnt[c] = c < ncells ? ((start[c + 1] - start[c] + 1) & ~1) : 0;
 }
 
+__global__ void grid_cell_tab_kernel(const int32_t* start, const int32_t* reach_start,
+                                     int64_t ncells, int4* tab) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) tab[c] = make_int4(start[c], start[c + 1], reach_start[c], 0);
+}
+
 // One thread per cell: sort the list by sphere index, then write the records
 // and the reach entries (make_records, make_reach_entry: the host builder's
 // arithmetic).
@@ -303,6 +309,10 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
     return fail(-1, "nz_substrate_create: reach entries exceed 2 GiB; lower cell_size or margin");
   uint2* reach = nullptr;
   if (int rc = dev_alloc(s, (size_t)rtotal + kReachPad, &reach)) return rc;
+  int4* cell_tab = nullptr;
+  if (int rc = dev_alloc(s, (size_t)ncells, &cell_tab)) return rc;
+  hipLaunchKernelGGL(grid_cell_tab_kernel, dim3(blocks256(ncells)), dim3(256), 0, st,
+                     (const int32_t*)start, (const int32_t*)reach_start, ncells, cell_tab);
   const int64_t rwords = 2 * ((int64_t)rtotal + kReachPad);
   hipLaunchKernelGGL(grid_zero_kernel, dim3(blocks256(rwords)), dim3(256), 0, st, (int32_t*)reach,
                      rwords);
@@ -335,6 +345,7 @@ int build_grid_device(const double* spheres, int64_t S, int32_t nstruct, const d
   g.reach = reach;
   g.reach_start = reach_start;
   g.reach_bytes = (int32_t)(sizeof(uint2) * ((size_t)rtotal + kReachPad));
+  g.cell_tab = cell_tab;
   g.sph = sph;
   g.sph_sid = sid;
   g.D = Dd;

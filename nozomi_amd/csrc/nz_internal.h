@@ -70,6 +70,9 @@ struct Grid {
   const uint2* reach;
   const int32_t* reach_start;  // [nloc^3]
   int32_t reach_bytes;         // size of reach in bytes (< 2^31)
+  // per cell (cell_start[c], cell_start[c + 1], reach_start[c], 0): what a
+  // sub-step needs of its cell in one 16-byte load
+  const int4* cell_tab;        // [nloc^3]
   int32_t nstruct;     // number of structures; the box has id nstruct
   int64_t nspheres;
   // CSR candidate lists over the nloc^3 local cells; entry e is the 32-byte

@@ -152,6 +152,7 @@ template <bool kTrack>
 __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, int64_t ld,
                                           int64_t i, bool active, const StepParams P,
                                           double* smom) {
+  const ListBufs lb = list_bufs(g);  // every lane is active here
   const int64_t ii = active ? i : 0;
   Lane s;
 #pragma unroll
@@ -201,7 +202,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
         Ga = P.garea[t - P.base];
       }
       sched_fence();
-      if (substep(g, s, q, P.seed, spin, step, P.dt, do_t2)) {
+      if (substep(g, lb, s, q, P.seed, spin, step, P.dt, do_t2)) {
         step_end(g, s, G, Ga);
         if (kTrack && P.rec) {  // displacement since the start, for the blocked phase contraction
           // slot-major: a lane's steps are contiguous (the lanes of a

@@ -350,13 +350,34 @@ NZ_HD inline int cand_sphere(const Grid& g, int e) {
 // Keeps the instruction scheduler from moving code across this point (the
 // loads issued above stay ahead of the arithmetic below).
 __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+// A buffer resource over [p, p + bytes).  Both are the same on every lane
+// (one grid per launch); taking them from the first lane keeps the resource
+// in scalar registers where the compiler cannot prove that itself (it would
+// otherwise loop over the distinct resources of the wavefront at every load).
+NZ_HD inline __amdgpu_buffer_rsrc_t uniform_rsrc(const void* p, int32_t bytes) {
+  const uint64_t v = (uint64_t)p;
+  // readfirstlane returns int: widen through uint32_t (a sign extension of
+  // the low word would overwrite the high word when bit 31 is set)
+  const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return __builtin_amdgcn_make_buffer_rsrc((void*)((hi << 32) | lo), (short)0,
+                                           __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
 typedef unsigned int nz_u32x4 __attribute__((ext_vector_type(4)));
+// The buffer resources of the candidate records and the reach entries,
+// built once per kernel where every lane is active (they stay in scalar
+// registers; built inside the scan, their loads and first-lane reads would
+// sit on every sub-step's critical path).
+struct ListBufs {
+  __amdgpu_buffer_rsrc_t cand, reach;
+};
+NZ_HD inline ListBufs list_bufs(const Grid& g) {
+  return ListBufs{uniform_rsrc(g.cand, g.cand_bytes), uniform_rsrc(g.reach, g.reach_bytes)};
+}
 struct RecLoader {
   __amdgpu_buffer_rsrc_t r;
   int base;  // byte offset of the list's first record
-  __device__ RecLoader(const Grid& g, int cs)
-      : r(__builtin_amdgcn_make_buffer_rsrc((void*)g.cand, (short)0, g.cand_bytes, 0x00020000)),
-        base(cs * 32) {}
+  __device__ RecLoader(const ListBufs& b, int cs) : r(b.cand), base(cs * 32) {}
   __device__ float4 ld(int voff, int soff) const {
     const nz_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
     return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
@@ -372,9 +393,7 @@ struct RecLoader {
 struct ReachLoader {
   __amdgpu_buffer_rsrc_t r;
   int base;  // byte offset of the list's first entry (16-byte aligned)
-  __device__ ReachLoader(const Grid& g, int rs)
-      : r(__builtin_amdgcn_make_buffer_rsrc((void*)g.reach, (short)0, g.reach_bytes, 0x00020000)),
-        base(rs * 8) {}
+  __device__ ReachLoader(const ListBufs& b, int rs) : r(b.reach), base(rs * 8) {}
   // entries e and e + 1 (e even, the same on every active lane: the loop
   // counter of the scan, taken from the first lane so the offset is scalar)
   __device__ nz_u32x4 ld2(int e) const {
@@ -384,9 +403,14 @@ struct ReachLoader {
 };
 #else
 NZ_HD inline void sched_fence() {}
+struct ListBufs {
+  const float4* cand;
+  const uint2* reach;
+};
+NZ_HD inline ListBufs list_bufs(const Grid& g) { return ListBufs{g.cand, g.reach}; }
 struct RecLoader {
   const float4* p;
-  RecLoader(const Grid& g, int cs) : p(g.cand + 2 * (int64_t)cs) {}
+  RecLoader(const ListBufs& b, int cs) : p(b.cand + 2 * (int64_t)cs) {}
   float4 a_u(int e) const { return p[2 * e]; }
   float4 a(int e) const { return p[2 * e]; }
   float4 b(int e) const { return p[2 * e + 1]; }
@@ -396,7 +420,7 @@ struct Words4 {
 };
 struct ReachLoader {
   const uint2* p;
-  ReachLoader(const Grid& g, int rs) : p(g.reach + rs) {}
+  ReachLoader(const ListBufs& b, int rs) : p(b.reach + rs) {}
   Words4 ld2(int e) const { return Words4{p[e].x, p[e].y, p[e + 1].x, p[e + 1].y}; }
 };
 #endif
@@ -479,15 +503,15 @@ NZ_HD inline int pop_flag(uint64_t& m, int none) {
 // (the skipped obstacles'), which cannot decide the sub-step, and the guard
 // count by out-of-reach spheres, whose FP64 re-solve could not change it
 // either.  Lists longer than 64 take the full visit for every candidate.
-NZ_HD inline void scan_list(const Grid& g, int cs, int ce, int rs, const ScanIn& in,
-                            ScanKeys& k) {
+NZ_HD inline void scan_list(const Grid& g, const ListBufs& lb, int cs, int ce, int rs,
+                            const ScanIn& in, ScanKeys& k) {
   const int n = ce - cs;
   if (n > 64) {
     scan_all(g.cand + 2 * (int64_t)cs, n, in, k);
     return;
   }
-  const RecLoader rl(g, cs);
-  const ReachLoader ql(g, rs);
+  const RecLoader rl(lb, cs);
+  const ReachLoader ql(lb, rs);
   // eight entries (four 16-byte loads) per group, two groups alternating:
   // one group's loads are in flight while the other is tested, and the
   // register sets swap roles, so nothing is copied.  Positions past the list
@@ -511,19 +535,22 @@ NZ_HD inline void scan_list(const Grid& g, int cs, int ce, int rs, const ScanIn&
     if (e >= n) break;
   }
   m &= n >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << n) - 1u);
-  // the marked records, each loaded while the previous one is visited; the
-  // position n (none left) loads the record after the list, not visited
+  // the marked records, each loaded while the previous one is visited, into
+  // two register sets that alternate (nothing is copied); the position n
+  // (none left) loads the record after the list, not visited
   int q = pop_flag(m, n);
   float4 A = rl.a(q);
   float4 B = rl.b(q);
   while (q < n) {
-    const int qn = pop_flag(m, n);
-    const float4 An = rl.a(qn);
-    const float4 Bn = rl.b(qn);
+    const int q2 = pop_flag(m, n);
+    const float4 A2 = rl.a(q2);
+    const float4 B2 = rl.b(q2);
     visit(in, A, B, (uint32_t)(q + 1), k);
-    q = qn;
-    A = An;
-    B = Bn;
+    if (q2 >= n) break;
+    q = pop_flag(m, n);
+    A = rl.a(q);
+    B = rl.b(q);
+    visit(in, A2, B2, (uint32_t)(q2 + 1), k);
   }
 }
 
@@ -659,6 +686,17 @@ NZ_HD inline int cell_of(const Grid& g, int a, double x) {
   return c < 0 ? 0 : (c > g.ncell[a] + 1 ? g.ncell[a] + 1 : c);
 }
 
+// v as a value the compiler cannot see through: the redraw of the first
+// jump after a structure change depends on nothing computed in the sub-step,
+// and without this the compiler evaluates its normals speculatively on every
+// sub-step (about 230 VALU) instead of in the rare branch that needs them.
+NZ_HD inline uint32_t opaque_u32(uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __asm__("" : "+v"(v));
+#endif
+  return v;
+}
+
 NZ_HD inline void draw_jump(uint64_t seed, uint32_t spin, uint32_t step, uint32_t draw, double ds,
                             double j[3]) {
   float n0, n1, n2;
@@ -711,10 +749,10 @@ NZ_HD inline void step_begin(const Grid& g, const Lane& s, StepState& q, double 
     q.ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
     q.ds_sid = s.sid;
   }
-  const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
-  q.cs = g.cell_start[cid];
-  q.ce = g.cell_start[cid + 1];
-  q.rs = g.reach_start[cid];
+  const int4 t = g.cell_tab[cell_id(g, s.l[0], s.l[1], s.l[2])];
+  q.cs = t.x;
+  q.ce = t.y;
+  q.rs = t.z;
 }
 
 // Draw the pending jump: draw 0 at the start of a step, draw k after the
@@ -741,8 +779,8 @@ NZ_HD inline double scatter_remaining(double jr, double jc) {
 
 // One sub-step of the multi-collision loop (sim3d_kernel.cu:373-487).
 // Returns true when the step is complete (:373 loop condition).
-NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, uint32_t spin,
-                          uint32_t step, double dt, bool do_t2) {
+NZ_HD inline bool substep(const Grid& g, const ListBufs& lb, Lane& s, StepState& q, uint64_t seed,
+                          uint32_t spin, uint32_t step, double dt, bool do_t2) {
   q.niter++;
   double jc = q.jr;  // :376
   double rel[3];
@@ -752,10 +790,10 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
     int exact_mode = -1;
     if (!in_safety(g, rel, q.j, q.jr)) {
       if (rebin(g, rel, s.l)) {
-        const int cid = cell_id(g, s.l[0], s.l[1], s.l[2]);
-        q.cs = g.cell_start[cid];
-        q.ce = g.cell_start[cid + 1];
-        q.rs = g.reach_start[cid];
+        const int4 t = g.cell_tab[cell_id(g, s.l[0], s.l[1], s.l[2])];
+        q.cs = t.x;
+        q.ce = t.y;
+        q.rs = t.z;
         if (!in_safety(g, rel, q.j, q.jr))
           exact_mode = in_neighbourhood(g, rel, q.j, q.jr, s.l) ? kExactNeighbours : kExactAll;
       } else {
@@ -769,7 +807,7 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
       scan_in(g, rel, q.j, q.jr, s.sid, in);
       ScanKeys k;
       keys_reset(g, k);
-      scan_list(g, q.cs, q.ce, q.rs, in, k);
+      scan_list(g, lb, q.cs, q.ce, q.rs, in, k);
       if (k.gcnt > 1) {
         exact_mode = kExactList;
       } else {
@@ -789,7 +827,7 @@ NZ_HD inline bool substep(const Grid& g, Lane& s, StepState& q, uint64_t seed, u
     if (q.niter == 1) {
       q.ds = sqrt(2.0 * g.D[(int)s.sid] * dt);
       q.ds_sid = s.sid;
-      draw_jump(seed, spin, step, 0u, q.ds, q.j);
+      draw_jump(seed, spin, step, opaque_u32(0u), q.ds, q.j);
     }
   }
   // The decisive roots in FP64 (the oracle's arithmetic).  A root whose FP32
@@ -885,7 +923,7 @@ NZ_HD inline void walk_step(const Grid& g, Lane& s, uint64_t seed, uint32_t spin
   bool done = false;
   while (!done) {
     if (q.pending >= 0) step_draw(q, seed, spin, step);
-    done = substep(g, s, q, seed, spin, step, dt, do_t2);
+    done = substep(g, list_bufs(g), s, q, seed, spin, step, dt, do_t2);
   }
   step_end(g, s, G, Ga);
 }

@@ -74,7 +74,7 @@ void nzcpu_scan_compare(void* h, int n, const double* pos, const double* jump, c
     ScanKeys k2, k1;
     keys_reset(g, k2);
     keys_reset(g, k1);
-    scan_list(g, cs, ce, g.reach_start[cid], in, k2);
+    scan_list(g, list_bufs(g), cs, ce, g.reach_start[cid], in, k2);
     scan_all(g.cand + 2 * (int64_t)cs, ce - cs, in, k1);
     out[4 * i + 0] = k2.kin != k1.kin;
     const float lim = (float)q.jr * 1.0001f + 1e-6f;
