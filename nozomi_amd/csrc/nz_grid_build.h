@@ -96,6 +96,13 @@ inline int grid_params(const double* spheres, int64_t S, int32_t nstruct, const 
     g.Ld[a] = L[a];
     hd[a] = L[a] / g.ncell[a];
   }
+  {
+    double hmax = 0.0;
+    for (int a = 0; a < 3; ++a) hmax = std::max(hmax, hd[a]);
+    const double shrink = 1e-5 * (hmax + margin);
+    g.safe_lo = (float)(-margin + shrink);
+    for (int a = 0; a < 3; ++a) g.safe_hi[a] = (float)(hd[a] + margin - shrink);
+  }
   M = margin + 4.0 * (double)kTol + 1e-5;
   // reach test parameters.  lambda = ell / R makes the bound exact; a
   // sub-step's segment is typically 0.4 of the list margin (the margin is

@@ -51,6 +51,10 @@ struct Grid {
   double hd[3];        // cell edge (FP64)
   double Ld[3];        // box edge (FP64)
   float margin;        // safety margin of the lists
+  // the safety box [-margin, h + margin] per axis shrunk by 1e-5 (h + margin):
+  // the FP32 test of a segment against it errs only towards "outside"
+  float safe_lo;
+  float safe_hi[3];
   // The reach test of the scan's first pass, per list entry (R = r + tol,
   // c the centre relative to the cell corner, ell the segment length, p the
   // position relative to the cell corner): a segment that touches the

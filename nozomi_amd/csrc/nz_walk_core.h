@@ -292,14 +292,16 @@ NZ_HD inline double cell_lo(const Grid& g, int a, int l) {
   return -0.5 * g.Ld[a] + (double)(l - 1) * g.hd[a];
 }
 
-// Segment p -> p + jr*j inside the safety box [-m, h+m]^3 of the lane's cell?
-NZ_HD inline bool in_safety(const Grid& g, const double p[3], const double j[3], double jr) {
+// Segment p -> p + jr*j (the FP32 scan inputs) inside the lane cell's safety
+// box, shrunk by far more than the FP32 round-off of p and p + jr j (below
+// 1e-6 of the cell scale): a segment it accepts is inside the box.
+NZ_HD inline bool in_safety_f(const Grid& g, const ScanIn& in, float jr) {
   bool ok = true;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double e = p[a] + jr * j[a];
-    const double lo = -(double)g.margin, hi = g.hd[a] + (double)g.margin;
-    ok = ok && p[a] >= lo && p[a] <= hi && e >= lo && e <= hi;
+    const float e = fmaf(jr, in.j[a], in.p[a]);
+    ok = ok & (in.p[a] >= g.safe_lo) & (in.p[a] <= g.safe_hi[a]) & (e >= g.safe_lo) &
+         (e <= g.safe_hi[a]);
   }
   return ok;
 }
@@ -729,11 +731,13 @@ NZ_HD inline void scan_in(const Grid& g, const double rel[3], const double j[3],
   }
   const float a = in.j[0] * in.j[0] + in.j[1] * in.j[1] + in.j[2] * in.j[2];
   in.a4 = 4.0f * a;
-  in.inv_a = 1.0f / a;
+  // 1/a serves the FP32 roots only (their decisions carry guard and tie
+  // bands far above the hardware reciprocal's error)
+  in.inv_a = nz_rcpf(a);
   in.sid = sid;
   // segment length with slack (relative 1e-3, absolute 1e-4 um) for the
   // FP32 round-off of its length
-  const float ell = sqrtf(a) * (float)jr * 1.001f + 1e-4f;
+  const float ell = nz_sqrtf(a) * (float)jr * 1.001f + 1e-4f;
   const float pp = fmaf(in.p[2], in.p[2], fmaf(in.p[1], in.p[1], in.p[0] * in.p[0]));
   in.rhs = fmaf(g.reach_lam1, ell * ell, g.reach_eps) - g.reach_sigma * pp;
 }
@@ -788,13 +792,16 @@ NZ_HD inline bool substep(const Grid& g, const ListBufs& lb, Lane& s, StepState&
   ScanAcc acc;
   for (int pass = 0; pass < 3; ++pass) {
     int exact_mode = -1;
-    if (!in_safety(g, rel, q.j, q.jr)) {
+    ScanIn in;
+    scan_in(g, rel, q.j, q.jr, s.sid, in);
+    if (!in_safety_f(g, in, (float)q.jr)) {
       if (rebin(g, rel, s.l)) {
         const int4 t = g.cell_tab[cell_id(g, s.l[0], s.l[1], s.l[2])];
         q.cs = t.x;
         q.ce = t.y;
         q.rs = t.z;
-        if (!in_safety(g, rel, q.j, q.jr))
+        scan_in(g, rel, q.j, q.jr, s.sid, in);
+        if (!in_safety_f(g, in, (float)q.jr))
           exact_mode = in_neighbourhood(g, rel, q.j, q.jr, s.l) ? kExactNeighbours : kExactAll;
       } else {
         exact_mode = kExactAll;
@@ -803,8 +810,6 @@ NZ_HD inline bool substep(const Grid& g, const ListBufs& lb, Lane& s, StepState&
     acc_reset(g, acc);
     if (exact_mode < 0 && q.ce - q.cs > kMaxListLen) exact_mode = kExactList;  // dense cell
     if (exact_mode < 0) {
-      ScanIn in;
-      scan_in(g, rel, q.j, q.jr, s.sid, in);
       ScanKeys k;
       keys_reset(g, k);
       scan_list(g, lb, q.cs, q.ce, q.rs, in, k);
