@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--box", type=float, default=60.0)
     ap.add_argument("--cell", type=float, default=0.0)
     ap.add_argument("--margin", type=float, default=0.0)
-    ap.add_argument("--sort", type=int, default=96, help="steps between spatial re-sorts")
+    ap.add_argument("--sort", type=int, default=250,
+                    help="steps between spatial re-sorts (250: 20 equal launches of the 5000-step walk)")
     ap.add_argument("--cpu-sample", type=int, default=200000,
                     help="spins of the CPU-baseline subsample (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=5000)

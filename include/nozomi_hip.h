@@ -128,7 +128,7 @@ typedef struct nz_walker* nz_walker_t;
 
 int nz_walker_create(nz_substrate_t sub, int64_t n, uint32_t flags, nz_walker_t* out);
 int nz_walker_destroy(nz_walker_t w);
-/* Steps between spatial re-sorts of the spins (0 = never sort). */
+/* Steps between spatial re-sorts of the spins (0 = never sort; default 256). */
 int nz_walker_set_sort_interval(nz_walker_t w, int32_t steps);
 /* Global index of the next step (the Philox step counter); set it to resume. */
 int nz_walker_get_step(nz_walker_t w, int64_t* step);

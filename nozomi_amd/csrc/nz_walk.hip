@@ -40,7 +40,7 @@ struct Walker {
   int32_t* partial = nullptr; // [scan tiles] tile sums of the key-count scan
   float* pos_tmp = nullptr;   // [3,n] seeding output
   int64_t nkeys = 0;          // sort key range: 2 x cells (intra / extra)
-  int32_t sort_interval = 96;
+  int32_t sort_interval = 256;
   int32_t xcd_chunk = 64;     // NZ_WALK_XCD_CHUNK in the environment overrides it
   int32_t since_sort = 1 << 30;
   int64_t step = 0;
@@ -184,8 +184,11 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   q.pending = -1;
   q.ds_sid = -1.0f;
   // A spin only stops at a step boundary: the sub-step state of a step in
-  // progress is not kept across launches.
-  while (__any(t < P.target || started)) {
+  // progress is not kept across launches.  (The wavefront's exit test sits
+  // at the bottom of the loop: tested at the top, the compiler copies the
+  // lane state in and out of the loop every iteration.)
+  bool go = __any(t < P.target);
+  while (go) {
     const bool behind = __any(t < P.target);
     if (active && (started || (t < P.limit && (t < P.target || behind)))) {
       const uint32_t step = t;
@@ -226,6 +229,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
         started = false;
       }
     }
+    go = __any(t < P.target || started);
   }
   if (kTrack && P.stepmom) {  // block partials -> FP64 sums
     __syncthreads();
