@@ -104,11 +104,12 @@ constexpr int kMomChunk = 256;
 // meanwhile spins that are ahead keep stepping up to `limit` (the end of the
 // walk), so collision-poor spins do not idle at every launch boundary.
 // Register budget of the walk kernel: waves per SIMD the compiler must fit
-// (0 = unconstrained).  Three waves (164 VGPRs, no spills) beat the
-// unconstrained two by 1.23x on the 200-axon substrate; four waves (128
-// VGPRs, 72 bytes per lane of spills) run 9% slower than three.
+// (0 = unconstrained).  On the 200-axon substrate (250-step launches over
+// 1e6 spins) four waves (128 VGPRs, 28 bytes per lane of spills) run 13.9 ms
+// per launch, three (140 VGPRs, none) 15.3 ms, five (96 VGPRs, 176 bytes of
+// spills) 20.7 ms.
 #ifndef NZ_WALK_WAVES_PER_EU
-#define NZ_WALK_WAVES_PER_EU 3
+#define NZ_WALK_WAVES_PER_EU 4
 #endif
 #if NZ_WALK_WAVES_PER_EU > 0
 #define NZ_WALK_ATTR __attribute__((amdgpu_waves_per_eu(NZ_WALK_WAVES_PER_EU)))
