@@ -119,7 +119,7 @@ constexpr int kMomChunk = 256;
 // The tracking walk (displacement record of the precision form, per-step
 // moments of the narrow-pulse driver) also holds the start position.
 #ifndef NZ_TRACK_WAVES_PER_EU
-#define NZ_TRACK_WAVES_PER_EU 3
+#define NZ_TRACK_WAVES_PER_EU 4
 #endif
 #if NZ_TRACK_WAVES_PER_EU > 0
 #define NZ_TRACK_ATTR __attribute__((amdgpu_waves_per_eu(NZ_TRACK_WAVES_PER_EU)))
