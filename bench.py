@@ -510,7 +510,7 @@ def serving_form(args):
     }
 
 
-WALK_SOURCES = ("nz_walk.hip", "nz_walk_core.h", "nz_internal.h", "nz_rng.h")
+WALK_SOURCES = ("nz_walk.hip", "nz_walk_core.h", "nz_internal.h", "nz_rng.h", "nz_grid_build.h")
 
 
 def walk_source_hash():
