@@ -23,10 +23,12 @@ from . import runtime as rt
 _CACHE = {}
 
 
-def tune(geom, key=None, n=65536, steps=128, dt=0.01, cells=(1.5, 1.75, 2.0, 2.25),
-         margins=(0.8, 0.9), ks=(64, 96, 128), device=None, seed=1):
+def tune(geom, key=None, n=65536, steps=512, dt=0.01, cells=(1.5, 1.75, 2.0, 2.25),
+         margins=(0.8, 0.9), ks=(96, 128, 256), device=None, seed=1):
     """Returns {"cell": c, "margin": m, "k": k, "ns_per_spin_step": t,
-    "table": [(t, c, m, k), ...]} for the geometry (a SimGeometry3D)."""
+    "table": [(t, c, m, k), ...]} for the geometry (a SimGeometry3D).  The
+    timed walk (steps) spans at least two sort intervals of every k, so each
+    candidate pays its re-sorts."""
     if key is not None and key in _CACHE:
         return _CACHE[key]
     device = rt._require_gpu(device)
