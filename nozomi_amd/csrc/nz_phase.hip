@@ -59,125 +59,240 @@ __global__ void __launch_bounds__(256) waveform_block_kernel(const float* g, int
   if (bt_f32) bt_f32[e] = v;
 }
 
-constexpr int kTileS = 64;   // slots per workgroup tile
-constexpr int kTileW = 64;   // waveforms per workgroup tile
-constexpr int kChunk = 32;   // k per MFMA (16x16x32 BF16)
-constexpr int kMaxK3 = 384;  // K <= 128: A operands held in registers
+typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-// phase[spin(s), w] += sum_k rec[s][k] * Bt[w][k] over one block of K3 =
-// 32 NKC rows, 16x16x32 BF16 MFMAs, split hi/lo (three products).
-// Workgroup: 4 waves and 64 slots; wave q owns slots [16q, 16q + 16) and
-// walks every waveform in tiles of 64.
-// - A (the block's displacements) is read from HBM once, straight into the
-//   registers that hold it as MFMA A operands for all waveform tiles (lane l:
-//   slot l % 16 of the wave, k = 8 (l / 16) .. + 8 of each 32-k chunk: 32
-//   contiguous bytes of the slot's record, the 4 lanes of a slot 128).
-// - B (the waveform block, a few hundred KB shared by all workgroups, served
-//   from L2) moves through two LDS buffers, one 32-k chunk of one waveform
-//   tile at a time: the next chunk's global load is in flight while the
-//   current chunk's MFMAs run, and one barrier per chunk suffices.
-// - D goes through LDS so each phase row is read-modified-written by 64 lanes
-//   at once (256 contiguous bytes); each wave touches only its own 16 rows.
-// 33 KB of LDS and 168 VGPRs: three workgroups per CU.
-// MFMA operand layout (lane l of 64): A row l % 16, k 8 (l / 16) .. + 8;
-// B column l % 16, the same k; D column l % 16, rows 4 (l / 16) .. + 4.
+// ---- split-BF16 contraction on 32x32x16 MFMAs, wave-specialised ----------
+//
+// phase[spin(s), w] += sum_k rec[s][k] * Bt[w][k] over one block of K3 = 3K
+// values per slot.  Workgroup: 128 slots (four groups of 32) and three kinds
+// of wave:
+//  - 4 KS MFMA waves: wave (g, q) holds the A operand (slots 32g .. 32g+31,
+//    k in part q of KS equal parts, split hi/lo) in registers for the whole
+//    kernel, and accumulates a 32 x 32 tile of (slot, waveform) sums with
+//    v_mfma_f32_32x32x16_bf16: hi.hi + hi.lo + lo.hi per 16-k step;
+//  - one waveform wave streams the waveform block (L2-resident) into an LDS
+//    ring of kRing entries with global_load_lds, entry = the 32-k chunk of
+//    every part for one waveform tile, issued kRing - 1 chunks ahead;
+//  - one phase wave streams the old phases of each tile (32 waveforms x 128
+//    rows, gathered by spin) into LDS one tile ahead.
+// Each wave's loads retire in issue order (vmcnt), so HBM latency (the phase
+// rows) and L2 latency (the waveform chunks) sit in different waves, and the
+// MFMA waves issue no loads after their A operand.  After a tile's chunks the
+// parts add their tiles into the old phases in LDS in a fixed order (part 0
+// first), and the last part stores the rows back, 128 contiguous bytes per
+// row.  Barrier per chunk interval (s_barrier, every wave the same count).
+// Measured at 4e6 slots x 1024 waveforms: v_mfma_f32_32x32x16_bf16 sustains
+// ~2.0 PFLOP/s against 1.2 for v_mfma_f32_16x16x32_bf16, and the 32-row tile
+// halves the LDS reads per MFMA.
+// MFMA operand layout (lane l of 64): A row l % 32, k 8 (l / 32) .. + 8;
+// B column l % 32, the same k; D column l % 32, rows 8 (v / 4) + 4 (l / 32)
+// + v % 4 for v = 0..15.
+constexpr int kTileS = 128;  // slots per workgroup
+constexpr int kTileW = 32;   // waveforms per tile
+constexpr int kChunk = 32;   // k per chunk interval (two 16-k MFMA steps)
+constexpr int kRing = 6;     // waveform chunk entries in LDS
+
 // Position of k (a multiple of 8) in LDS row `col` of a waveform chunk: rows
-// of 32 BF16 (64 bytes, four 16-byte pieces) with the piece index XORed with
-// bits 1-2 of the row, so that the MFMA B reads (16 rows, one piece each) and
-// the chunk stores (4 pieces of 16 rows) fall in distinct LDS banks (the
-// bank-conflict counter reads ~0 against 44% of LDS cycles with padded rows).
-__device__ __forceinline__ int sbk(int col, int k) { return 8 * ((k >> 3) ^ ((col >> 1) & 3)); }
+// of 32 BF16 (64 bytes, four 16-byte pieces), piece index XORed with bits 2-3
+// of the row, so the 32 rows x 2 pieces of an MFMA B read use every bank
+// pair-wise (the minimum for 1 KB) and the ring stores (16 rows x 4 pieces)
+// likewise.
+__device__ __forceinline__ int bpiece(int col, int k) { return 8 * ((k >> 3) ^ ((col >> 2) & 3)); }
+// Word of (row r, column c) of a 128 x 32 FP32 phase tile in LDS: rows of 32
+// words, the 4-word group index XORed with 4 ((r / 4) % 2).  Phase-wave load
+// j (0..15) of lane l covers words 256 j + 4 l .. + 4: row 8 j + l / 8,
+// columns 4 ((l % 8) ^ 4 ((l / 32) % 2)) .. + 4, and the MFMA waves store
+// the same pieces back.
+__device__ __forceinline__ int pword(int r, int c) {
+  return r * kTileW + ((((c >> 2) ^ (((r >> 2) & 1) << 2))) << 2) + (c & 3);
+}
+__device__ __forceinline__ int ppiece_col(int lane) { return ((lane & 7) ^ (4 * ((lane >> 5) & 1))) << 2; }
 
-template <int NKC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+// Wait until at most n (a multiple of 4, <= 60) vector memory operations of
+// this wave are outstanding.
+__device__ __forceinline__ void wait_vmcnt(int n) {
+  switch (n >> 2) {
+    case 0: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    case 1: asm volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
+    case 2: asm volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
+    case 3: asm volatile("s_waitcnt vmcnt(12)" ::: "memory"); break;
+    case 4: asm volatile("s_waitcnt vmcnt(16)" ::: "memory"); break;
+    case 5: asm volatile("s_waitcnt vmcnt(20)" ::: "memory"); break;
+    case 6: asm volatile("s_waitcnt vmcnt(24)" ::: "memory"); break;
+    case 7: asm volatile("s_waitcnt vmcnt(28)" ::: "memory"); break;
+    case 8: asm volatile("s_waitcnt vmcnt(32)" ::: "memory"); break;
+    case 9: asm volatile("s_waitcnt vmcnt(36)" ::: "memory"); break;
+    case 10: asm volatile("s_waitcnt vmcnt(40)" ::: "memory"); break;
+    case 11: asm volatile("s_waitcnt vmcnt(44)" ::: "memory"); break;
+    case 12: asm volatile("s_waitcnt vmcnt(48)" ::: "memory"); break;
+    case 13: asm volatile("s_waitcnt vmcnt(52)" ::: "memory"); break;
+    case 14: asm volatile("s_waitcnt vmcnt(56)" ::: "memory"); break;
+    default: asm volatile("s_waitcnt vmcnt(60)" ::: "memory"); break;
+  }
+}
+// LDS writes done, then the workgroup barrier.
+__device__ __forceinline__ void wg_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// NKC chunk intervals per tile, KS parts of K: K3 = 32 NKC KS.
+template <int NKC, int KS>
+__global__ void __launch_bounds__(64 * (4 * KS + 2))
+__attribute__((amdgpu_waves_per_eu(KS == 3 ? 4 : 3)))
 phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin,
                        const uint16_t* bt_hi, const uint16_t* bt_lo, int32_t W, float* phase) {
-  constexpr int K3 = NKC * kChunk;
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][kTileW][kChunk];  // [buf][hi, lo]
-  __shared__ float sC[kTileS][kTileW + 1];
+  constexpr int KP = NKC * kChunk;  // k per part
+  constexpr int K3 = KP * KS;
+  constexpr int kEntry = KS * 2 * kTileW * kChunk;  // uint16 per ring entry
+  constexpr int kTileWords = kTileS * kTileW;
+  __shared__ __attribute__((aligned(16))) uint16_t sB[kRing * kEntry];
+  __shared__ __attribute__((aligned(16))) float sP[2 * kTileWords];
   const int64_t s0 = (int64_t)blockIdx.x * kTileS;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int kl = 8 * (lane >> 4);
-  const int64_t srow = s0 + wave * 16 + (lane & 15);
-  const bool row_ok = srow < nslots;
-  bf16x8 ah[NKC], al[NKC];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ntiles = (W + kTileW - 1) / kTileW;
+  const int nint = ntiles * NKC;  // chunk intervals
+
+  if (wave < 4 * KS) {  // ---- MFMA wave (slot group sg, part kq)
+    const int sg = wave & 3, kq = wave >> 2;
+    const int kh = 8 * (lane >> 5), col = lane & 31;
+    bf16x8 ah[2 * NKC], al[2 * NKC];
+    {
+      const int64_t srow = s0 + sg * 32 + col;
+      const bool row_ok = srow < nslots;
+      const float* arow = rec + (row_ok ? srow : 0) * ld + kq * KP + kh;
 #pragma unroll
-  for (int kc = 0; kc < NKC; ++kc) {
-    uint16_t hb[8], lb[8];
+      for (int ks = 0; ks < 2 * NKC; ++ks) {
+        floatx4 v[2];
+        __builtin_memcpy(&v[0], arow + ks * 16, 16);
+        __builtin_memcpy(&v[1], arow + ks * 16 + 4, 16);
+        uint16_t hb[8], lb[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float v = row_ok ? rec[srow * ld + kc * kChunk + kl + j] : 0.0f;
-      hb[j] = bf16_bits(v);
-      lb[j] = bf16_bits(v - bf16_float(hb[j]));
+        for (int j = 0; j < 8; ++j) {
+          const float x = row_ok ? v[j >> 2][j & 3] : 0.0f;
+          hb[j] = bf16_bits(x);
+          lb[j] = bf16_bits(x - bf16_float(hb[j]));
+        }
+        __builtin_memcpy(&ah[ks], hb, 16);
+        __builtin_memcpy(&al[ks], lb, 16);
+      }
     }
-    __builtin_memcpy(&ah[kc], hb, 16);
-    __builtin_memcpy(&al[kc], lb, 16);
-  }
-  // the wave's 16 slots -> spins
-  int32_t spin_row = -1;
-  if (lane < 16) {
-    const int64_t s = s0 + wave * 16 + lane;
-    spin_row = (s < nslots) ? slot_spin[s] : -1;
-  }
-  // this thread's 16-byte piece of every B chunk: waveform ww, k q .. q + 8
-  const int ww = tid >> 2, q = (tid & 3) * 8;
-  bf16x8 nh = {}, nl = {};  // the next chunk, in flight
-  if (ww < W) {
-    __builtin_memcpy(&nh, &bt_hi[(int64_t)ww * K3 + q], 16);
-    __builtin_memcpy(&nl, &bt_lo[(int64_t)ww * K3 + q], 16);
-  }
-  int buf = 0;
-  for (int w0 = 0; w0 < W; w0 += kTileW) {
-    floatx4 acc[4];
+    int32_t spin_row;
+    {
+      const int64_t s = s0 + sg * 32 + col;
+      const int32_t v = slot_spin[s < nslots ? s : nslots - 1];
+      spin_row = (s < nslots) ? v : -1;
+    }
+    int32_t psp[4];  // spins of this lane's four store pieces (rows 32 sg + 8 j + l / 8)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < 4; ++j) psp[j] = __shfl(spin_row, 8 * j + (lane >> 3), 64);
+    const int pcol = ppiece_col(lane);
+    const int bo0 = col * kChunk + bpiece(col, kh), bo1 = col * kChunk + bpiece(col, 16 + kh);
+    wg_barrier();
+    int g = 0;
+    for (int t = 0; t < ntiles; ++t) {
+      floatx16 acc;
 #pragma unroll
-    for (int kc = 0; kc < NKC; ++kc) {
-      // buffer `buf` was last read two chunks ago, before the previous barrier
-      __builtin_memcpy(&sB[buf][0][ww][sbk(ww, q)], &nh, 16);
-      __builtin_memcpy(&sB[buf][1][ww][sbk(ww, q)], &nl, 16);
-      {
-        const int nw0 = (kc + 1 < NKC) ? w0 : w0 + kTileW;
-        const int nkc = (kc + 1 < NKC) ? kc + 1 : 0;
-        if (nw0 < W) {
-          nh = bf16x8{};
-          nl = bf16x8{};
-          if (nw0 + ww < W) {
-            const int64_t o = (int64_t)(nw0 + ww) * K3 + nkc * kChunk + q;
-            __builtin_memcpy(&nh, &bt_hi[o], 16);
-            __builtin_memcpy(&nl, &bt_lo[o], 16);
+      for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+#pragma unroll
+      for (int kc = 0; kc < NKC; ++kc, ++g) {
+        const uint16_t* bh = sB + (g % kRing) * kEntry + kq * (2 * kTileW * kChunk);
+        const uint16_t* bl = bh + kTileW * kChunk;
+        bf16x8 h0, l0, h1, l1;
+        __builtin_memcpy(&h0, bh + bo0, 16);
+        __builtin_memcpy(&l0, bl + bo0, 16);
+        __builtin_memcpy(&h1, bh + bo1, 16);
+        __builtin_memcpy(&l1, bl + bo1, 16);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[2 * kc], h0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[2 * kc], l0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[2 * kc], h0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[2 * kc + 1], h1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[2 * kc + 1], l1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[2 * kc + 1], h1, acc, 0, 0, 0);
+        wg_barrier();
+      }
+      float* tp = sP + (t & 1) * kTileWords;
+#pragma unroll
+      for (int q = 0; q < KS; ++q) {
+        if (kq == q) {
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            tp[pword(sg * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), col)] += acc[v];
+          if (q == KS - 1) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              floatx4 v;
+              __builtin_memcpy(&v, tp + (sg * 32 + 8 * j) * kTileW + 4 * lane, 16);
+              const int c = t * kTileW + pcol;
+              if (psp[j] >= 0 && c < W) __builtin_memcpy(phase + (int64_t)psp[j] * W + c, &v, 16);
+            }
           }
         }
+        wg_barrier();
       }
-      __syncthreads();
+    }
+  } else if (wave == 4 * KS) {  // ---- waveform wave: ring entries
+    // entry of interval gi: chunk gi % NKC of tile gi / NKC, all KS parts
+    auto issue = [&](int gi) {
+      const int t = gi / NKC, kc = gi - t * NKC;
+      uint16_t* dst = sB + (gi % kRing) * kEntry;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int col = c * 16 + (lane & 15);
-        bf16x8 bh, bl;
-        __builtin_memcpy(&bh, &sB[buf][0][col][sbk(col, kl)], 16);
-        __builtin_memcpy(&bl, &sB[buf][1][col][sbk(col, kl)], 16);
-        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bh, acc[c], 0, 0, 0);
-        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kc], bl, acc[c], 0, 0, 0);
-        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kc], bh, acc[c], 0, 0, 0);
+      for (int q = 0; q < KS; ++q)
+#pragma unroll
+        for (int hl = 0; hl < 2; ++hl)
+#pragma unroll
+          for (int cg = 0; cg < 2; ++cg) {
+            const int c = cg * 16 + (lane >> 2);
+            const int kp = (lane & 3) ^ ((c >> 2) & 3);  // LDS piece lane & 3 holds k-piece kp
+            const int w = t * kTileW + c;
+            const uint16_t* src = (hl ? bt_lo : bt_hi) + (int64_t)(w < W ? w : 0) * K3 + q * KP +
+                                  kc * kChunk + 8 * kp;
+            __builtin_amdgcn_global_load_lds(src, dst + (q * 2 + hl) * (kTileW * kChunk) + cg * 16 * kChunk,
+                                             16, 0, 0);
+          }
+    };
+    const int first = nint < kRing ? nint : kRing;
+    for (int gi = 0; gi < first; ++gi) issue(gi);
+    wait_vmcnt(4 * KS * (first - 1));  // entry 0 landed
+    wg_barrier();
+    int g = 0;
+    for (int t = 0; t < ntiles; ++t) {
+      for (int kc = 0; kc < NKC; ++kc, ++g) {
+        // the entry of interval g - 1 was read before the last barrier
+        if (g >= 1 && g + kRing - 1 < nint) issue(g + kRing - 1);
+        if (g + 1 < nint) {  // entry g + 1 lands before the barrier that opens it
+          const int last = (g + kRing - 1 < nint) ? g + kRing - 1 : nint - 1;
+          wait_vmcnt(4 * KS * (last - (g + 1)));
+        }
+        wg_barrier();
       }
-      buf ^= 1;
+      for (int q = 0; q < KS; ++q) wg_barrier();
     }
+  } else {  // ---- phase wave: old phases of tile t into sP[t % 2]
+    int32_t sp16[16];  // spins of rows 8 j + l / 8 (padding slots read spin 0, never stored)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) sC[wave * 16 + 4 * (lane >> 4) + i][c * 16 + (lane & 15)] = acc[c][i];
-    __syncthreads();
-    const int w = w0 + lane;
-    float old[16];  // all 16 row loads in flight before the first add
-#pragma unroll
-    for (int rr = 0; rr < 16; ++rr) {
-      const int32_t sp = __shfl(spin_row, rr, 64);
-      old[rr] = (sp >= 0 && w < W) ? phase[(int64_t)sp * W + w] : 0.0f;
+    for (int j = 0; j < 16; ++j) {
+      const int64_t s = s0 + 8 * j + (lane >> 3);
+      const int32_t v = slot_spin[s < nslots ? s : nslots - 1];
+      sp16[j] = (s < nslots && v >= 0) ? v : 0;
     }
+    const int pcol = ppiece_col(lane);
+    auto issue = [&](int t) {
+      float* dst = sP + (t & 1) * kTileWords;
+      const int c = t * kTileW + pcol;
+      const int cc = c < W ? c : t * kTileW;  // W % 4 == 0: a piece is all in or all out
 #pragma unroll
-    for (int rr = 0; rr < 16; ++rr) {
-      const int32_t sp = __shfl(spin_row, rr, 64);
-      if (sp >= 0 && w < W) phase[(int64_t)sp * W + w] = old[rr] + sC[wave * 16 + rr][lane];
+      for (int j = 0; j < 16; ++j)
+        __builtin_amdgcn_global_load_lds(phase + (int64_t)sp16[j] * W + cc, dst + 256 * j, 16, 0, 0);
+    };
+    issue(0);
+    if (ntiles > 1) issue(1);
+    wg_barrier();
+    for (int t = 0; t < ntiles; ++t) {
+      for (int kc = 0; kc < NKC; ++kc) {
+        // sP[(t + 1) % 2] was last read in tile t - 1's epilogue
+        if (kc == 0 && t >= 1 && t + 1 < ntiles) issue(t + 1);
+        if (kc == NKC - 1) wait_vmcnt(t + 1 < ntiles ? 16 : 0);  // tile t landed
+        wg_barrier();
+      }
+      for (int q = 0; q < KS; ++q) wg_barrier();
     }
   }
 }
@@ -295,15 +410,23 @@ int nz_phase_gemm(const float* rec, int64_t ld, int64_t nslots, const int32_t* s
   if (nslots <= 0 || ld < K3 || W <= 0 || K <= 0) return fail(-1, "nz_phase_gemm: bad sizes");
   if (mode == 0) {
     if (!bt_hi || !bt_lo) return fail(-1, "nz_phase_gemm: MFMA mode needs bt_hi and bt_lo");
-    if (K3 % kChunk || K3 > kMaxK3)
-      return fail(-1, "nz_phase_gemm: MFMA mode needs K a multiple of 32 and at most 128");
+    if (K3 != 96 && K3 != 192 && K3 != 288 && K3 != 384)
+      return fail(-1, "nz_phase_gemm: MFMA mode needs K = 32, 64, 96 or 128");
+    if (W % 4 || ld % 4) return fail(-1, "nz_phase_gemm: MFMA mode needs W and ld multiples of 4");
     dim3 grid((unsigned)((nslots + kTileS - 1) / kTileS));
-    auto* kern = (K3 == 96)    ? phase_gemm_mfma_kernel<3>
-                 : (K3 == 192) ? phase_gemm_mfma_kernel<6>
-                 : (K3 == 288) ? phase_gemm_mfma_kernel<9>
-                               : phase_gemm_mfma_kernel<12>;
-    hipLaunchKernelGGL(kern, grid, dim3(256), 0, (hipStream_t)stream, rec, ld, nslots, slot_spin,
-                       bt_hi, bt_lo, W, phase);
+    // K = 32, 64: one part; 96: three parts of 32 steps; 128: two parts of 64
+    if (K3 == 96)
+      hipLaunchKernelGGL((phase_gemm_mfma_kernel<3, 1>), grid, dim3(64 * 6), 0, (hipStream_t)stream,
+                         rec, ld, nslots, slot_spin, bt_hi, bt_lo, W, phase);
+    else if (K3 == 192)
+      hipLaunchKernelGGL((phase_gemm_mfma_kernel<6, 1>), grid, dim3(64 * 6), 0, (hipStream_t)stream,
+                         rec, ld, nslots, slot_spin, bt_hi, bt_lo, W, phase);
+    else if (K3 == 288)
+      hipLaunchKernelGGL((phase_gemm_mfma_kernel<3, 3>), grid, dim3(64 * 14), 0, (hipStream_t)stream,
+                         rec, ld, nslots, slot_spin, bt_hi, bt_lo, W, phase);
+    else
+      hipLaunchKernelGGL((phase_gemm_mfma_kernel<6, 2>), grid, dim3(64 * 10), 0, (hipStream_t)stream,
+                         rec, ld, nslots, slot_spin, bt_hi, bt_lo, W, phase);
   } else {
     if (!bt_f32) return fail(-1, "nz_phase_gemm: FP32 mode needs bt_f32");
     dim3 grid((unsigned)((nslots + 255) / 256), (unsigned)((W + 15) / 16));

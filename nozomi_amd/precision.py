@@ -68,10 +68,17 @@ class PrecisionRun:
     def __init__(self, sub, x0, seed, g, u, dt, K=64, mode="mfma", overlap=True):
         self.device = sub.device
         self.N = x0.shape[1]
-        self.W, self.T = g.shape
         self.K = int(K)
-        if mode == "mfma" and (self.K % 32 or self.K > 128):
-            raise ValueError("the MFMA contraction needs K a multiple of 32, at most 128")
+        if mode == "mfma" and self.K not in (32, 64, 96, 128):
+            raise ValueError("the MFMA contraction needs K = 32, 64, 96 or 128")
+        # the MFMA contraction moves phase rows in 16-byte pieces: W is padded
+        # to a multiple of 4 with zero waveforms (phases stay 0, not reported)
+        self.W_out = g.shape[0]
+        pad = (-g.shape[0]) % 4
+        if pad:
+            g = np.concatenate([g, np.zeros((pad, g.shape[1]), g.dtype)])
+            u = np.concatenate([u, np.zeros((3, pad), u.dtype)], axis=1)
+        self.W, self.T = g.shape
         self.mode = 0 if mode == "mfma" else 1
         self.seed, self.dt = int(seed), float(dt)
         d = self.device
@@ -167,7 +174,7 @@ class PrecisionRun:
         with torch.cuda.device(self.device):
             check(lib().nz_phase_signals(_ptr(self.phase), self.N, self.W, _ptr(self.signals_out),
                                          _stream_ptr(self.device)), "nz_phase_signals")
-        s = self.signals_out.cpu().numpy()
+        s = self.signals_out.cpu().numpy()[:self.W_out]
         return (s[:, 0] + 1j * s[:, 1]) / self.N
 
     def close(self):

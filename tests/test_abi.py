@@ -49,6 +49,14 @@ def test_argument_errors_are_reported():
     rc = L.nz_batch_add(None, None, 10, 0, 10, 0.01, None, None, 0, 64, None, None, 0,
                         ctypes.byref(sim), None)
     assert rc < 0 and b"batch" in L.nz_last_error_string()
+    # the MFMA contraction's shape rules are checked before any launch (the
+    # buffers below are host memory and are never touched)
+    buf = ctypes.create_string_buffer(64)
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    rc = L.nz_phase_gemm(p, 192, 128, p, 64, p, p, None, 50, 0, p, None)
+    assert rc < 0 and b"multiples of 4" in L.nz_last_error_string()
+    rc = L.nz_phase_gemm(p, 240, 128, p, 80, p, p, None, 64, 0, p, None)
+    assert rc < 0 and b"K = 32, 64, 96 or 128" in L.nz_last_error_string()
 
 
 def test_host_tensors_are_refused_before_any_launch():

@@ -49,6 +49,28 @@ def test_gemm_modes_against_fp64(K):
         assert np.array_equal(got[untouched], ph0[untouched].astype(np.float64))
 
 
+def test_waveform_count_padded_to_four():
+    """W = 50 (not a multiple of 4): the run pads two zero waveforms for the
+    MFMA contraction and reports 50 signals, equal to the FP32 contraction's
+    within the split-BF16 bound."""
+    M, L = generate_beaded_axons(16, 30.0, seed=20261203)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    sub = geom.device_substrate(device=DEV)
+    N, T, dt, W, K = 8192, 96, 0.02, 50, 32
+    g, u, _ = precision_waveforms(W, T, dt, seed=5)
+    x0 = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=3)
+    out = {}
+    for mode in ("mfma", "fp32"):
+        pr = PrecisionRun(sub, x0, 17, g, u, dt, K=K, mode=mode)
+        pr.run()
+        out[mode] = pr.signals()
+        assert pr.W == 52 and out[mode].shape == (W,)
+        pr.close()
+    d = np.abs(out["mfma"] - out["fp32"]).max()
+    print(f"W = 50: max |signal MFMA - FP32| = {d:.1e}")
+    assert d < 1e-5
+
+
 def test_precision_walk_against_oracle(oracle_lib):
     M, L = generate_beaded_axons(16, 30.0, seed=20261203)
     geom = load_geometry(M, L, D=2.0, T2=100.0)
