@@ -254,8 +254,8 @@ int nz_walk_record(nz_walker_t w, uint64_t seed, int32_t nsteps, double dt, floa
 int nz_waveform_block(const float* g, int64_t T, const float* u, int32_t W, int64_t t0, int32_t K,
                       uint16_t* bt_hi, uint16_t* bt_lo, float* bt_f32, void* stream);
 /* phase [n_spins][W] += rec . Bt^T over the block, rec [nslots][ld] as
- * nz_walk_record writes it (ld >= 3K): mode 0 = split hi/lo BF16 MFMA (K a
- * multiple of 32, at most 128), mode 1 = FP32 vector ALUs. */
+ * nz_walk_record writes it (ld >= 3K): mode 0 = split hi/lo BF16 MFMA (K =
+ * 32, 64, 96 or 128; W and ld multiples of 4), mode 1 = FP32 vector ALUs. */
 int nz_phase_gemm(const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin,
                   int32_t K, const uint16_t* bt_hi, const uint16_t* bt_lo, const float* bt_f32,
                   int32_t W, int32_t mode, float* phase, void* stream);
