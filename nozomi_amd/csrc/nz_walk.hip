@@ -47,7 +47,7 @@ struct Walker {
   // instead of waiting for the three others of a 256-thread workgroup (200-axon
   // substrate, 1e6 spins, 250-step launches: 13.37 ms against 13.95 ms).
   int32_t walk_block = 64;
-  // threads per track_kernel workgroup (NZ_TRACK_BLOCK); 256 measured faster
+  // threads per moments_kernel / record_kernel workgroup (NZ_TRACK_BLOCK); 256 measured faster
   // here than 64 (the per-step moment partials are per workgroup)
   int32_t track_block = 256;
   int32_t since_sort = 1 << 30;
@@ -152,12 +152,16 @@ struct StepParams {
   int64_t rec_ld;    // floats per slot of the record (>= 3 * (target - mom_base))
 };
 
+// What a walk launch keeps beside the walk itself.  The tracking kinds hold
+// the start position in registers; each kind is its own kernel, so the plain
+// walk and the record leave the other kinds' registers to the scan.
+enum WalkKind { kPlain = 0, kMoments = 1, kRecord = 2 };
+
 // Walk the spin held in slot i (fields strided by ld) from its next step to
 // at least P.target.  Every lane of the block calls it (inactive lanes too:
-// the per-step moment partials synchronise the block).  kTrack: the launch
-// records displacements or per-step moments, which keep the start position
-// in registers; the plain walk leaves those registers to the scan.
-template <bool kTrack>
+// the per-step moment partials synchronise the block).  kMoments: per-step
+// FP64 moment sums (P.stepmom); kRecord: per-step displacement record (P.rec).
+template <int kKind>
 __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, int64_t ld,
                                           int64_t i, bool active, const StepParams P,
                                           double* smom) {
@@ -179,11 +183,11 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   const uint32_t spin = active ? (uint32_t)st.spin_id[ii] : 0u;
   uint32_t t = active ? st.tstep[ii] : P.target;
   double x0[3] = {0.0, 0.0, 0.0};
-  if (kTrack && P.stepmom) {
+  if (kKind == kMoments) {
     for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x) smom[k] = 0.0;
     __syncthreads();
   }
-  if (kTrack && (P.stepmom || P.rec)) {
+  if (kKind != kPlain) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) x0[a] = active ? st.x0[a * ld + ii] : 0.0;
   }
@@ -216,7 +220,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
       sched_fence();
       if (substep(g, lb, s, q, P.seed, spin, step, P.dt, do_t2)) {
         step_end(g, s, G, Ga);
-        if (kTrack && P.rec) {  // displacement since the start, for the blocked phase contraction
+        if (kKind == kRecord) {  // displacement since the start, for the blocked phase contraction
           // slot-major: a lane's steps are contiguous (the lanes of a
           // wavefront are at different steps, so rows of steps would scatter)
           float* r = P.rec + i * P.rec_ld + (int64_t)(t - P.mom_base) * 3;
@@ -224,7 +228,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
           r[1] = (float)lane_disp(g, s, x0, 1);
           r[2] = (float)lane_disp(g, s, x0, 2);
         }
-        if (kTrack && P.stepmom) {  // moment sums (sim3d_kernel.cu:191-237), FP64 throughout
+        if (kKind == kMoments) {  // moment sums (sim3d_kernel.cu:191-237), FP64 throughout
           const int k = (int)(t - P.mom_base) * 6;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
@@ -240,7 +244,7 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
     }
     go = __any(t < P.target || started);
   }
-  if (kTrack && P.stepmom) {  // block partials -> FP64 sums
+  if (kKind == kMoments) {  // block partials -> FP64 sums
     __syncthreads();
     for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x)
       atomicAdd(&P.stepmom[(int64_t)(P.mom_base - P.base) * 6 + k], smom[k]);
@@ -273,11 +277,11 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR walk_kernel(const WalkArgs A
   const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
                      (uint32_t)A.mom_base, A.dt, A.gph, A.garea, A.stepmom, A.flags, A.rec,
                      A.rec_ld};
-  walk_slot<false>(*gp, A.st, A.n, i, i < A.n, P, nullptr);
+  walk_slot<kPlain>(*gp, A.st, A.n, i, i < A.n, P, nullptr);
 }
 
-__global__ void __launch_bounds__(256) NZ_TRACK_ATTR track_kernel(const WalkArgs A,
-                                                                const Grid* __restrict__ gpr) {
+__global__ void __launch_bounds__(256) NZ_TRACK_ATTR moments_kernel(const WalkArgs A,
+                                                                  const Grid* __restrict__ gpr) {
   // per-step moment partials of the block (12 KiB at kMomChunk = 256)
   __shared__ double smom[kMomChunk * 6];
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
@@ -285,7 +289,17 @@ __global__ void __launch_bounds__(256) NZ_TRACK_ATTR track_kernel(const WalkArgs
   const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
                      (uint32_t)A.mom_base, A.dt, A.gph, A.garea, A.stepmom, A.flags, A.rec,
                      A.rec_ld};
-  walk_slot<true>(*gpr, A.st, A.n, i, i < A.n, P, smom);
+  walk_slot<kMoments>(*gpr, A.st, A.n, i, i < A.n, P, smom);
+}
+
+__global__ void __launch_bounds__(256) NZ_TRACK_ATTR record_kernel(const WalkArgs A,
+                                                                 const Grid* __restrict__ gpr) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x, A.xcd_chunk);
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
+                     (uint32_t)A.mom_base, A.dt, A.gph, A.garea, nullptr, A.flags, A.rec,
+                     A.rec_ld};
+  walk_slot<kRecord>(*gpr, A.st, A.n, i, i < A.n, P, nullptr);
 }
 
 // Absolute start positions (spin order) -> resident state (slot = spin).
@@ -815,7 +829,7 @@ __global__ void __launch_bounds__(256) NZ_WALK_ATTR batch_walk_kernel(const SimD
   const int64_t target = min(d.step + d.k, d.nsteps);
   const StepParams P{d.seed, 0u, (uint32_t)target, (uint32_t)d.nsteps, 0u, d.dt, d.gph, d.garea,
                      nullptr, d.flags, nullptr};
-  walk_slot<false>(d.g, st, ld, i, st.spin_id[i] >= 0, P, nullptr);
+  walk_slot<kPlain>(d.g, st, ld, i, st.spin_id[i] >= 0, P, nullptr);
 }
 
 // After a launch: every active simulation has advanced to its target.
@@ -1179,7 +1193,7 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
       NZ_HIP(hipEventRecord(e0, s));
     }
     if (stepmom)
-      hipLaunchKernelGGL(track_kernel, dim3(blocks_for(w->n, w->track_block)), dim3(w->track_block), 0, s, a,
+      hipLaunchKernelGGL(moments_kernel, dim3(blocks_for(w->n, w->track_block)), dim3(w->track_block), 0, s, a,
                          (const Grid*)w->sub->g_dev);
     else
       hipLaunchKernelGGL(walk_kernel, dim3(blocks_for(w->n, w->walk_block)), dim3(w->walk_block), 0,
@@ -1221,7 +1235,7 @@ int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, flo
   a.xcd_chunk = w->xcd_chunk;
   a.rec = rec;
   a.rec_ld = ld;
-  hipLaunchKernelGGL(track_kernel, dim3(blocks_for(w->n, w->track_block)), dim3(w->track_block), 0, s, a,
+  hipLaunchKernelGGL(record_kernel, dim3(blocks_for(w->n, w->track_block)), dim3(w->track_block), 0, s, a,
                      (const Grid*)w->sub->g_dev);
   NZ_HIP(hipGetLastError());
   NZ_HIP(hipMemcpyAsync(slot_spin, w->buf[w->cur].spin_id, sizeof(int32_t) * w->n,
