@@ -357,8 +357,10 @@ def _projection_args(mom, dirs, scale):
     if scale is not None and (scale.dtype != torch.float32 or scale.numel() != W):
         raise ValueError("scale must be a float32 tensor of W entries")
     dev = mom.device
-    return (_ptr(mom, dev, "mom"), _ptr(dirs.contiguous(), dev, "dirs"),
-            _ptr(scale, dev, "scale"), dirs.contiguous())
+    # one dense copy: its address goes to the kernel and the caller holds it
+    # until the launch is queued (a second .contiguous() would be another copy)
+    dc = dirs.contiguous()
+    return (_ptr(mom, dev, "mom"), _ptr(dc, dev, "dirs"), _ptr(scale, dev, "scale"), dc)
 
 
 def project_phase(mom, dirs, scale=None, out=None):

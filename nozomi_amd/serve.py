@@ -191,6 +191,14 @@ class Server:
             self.queue.append(req)
 
     # -- scheduling -------------------------------------------------------------
+    def _answer_error(self, req, e):
+        """The request cannot run: drop it from the queue and answer it."""
+        with self.lock:
+            self.queue.popleft()
+        now = time.perf_counter()
+        self.results[req.rid] = Result(req.rid, None, req.arrival, now, now,
+                                       error=f"{type(e).__name__}: {e}")
+
     def _admit(self):
         admitted = 0
         while True:
@@ -204,14 +212,18 @@ class Server:
             largest, _, free_sims = self.batch.free()
             if free_sims == 0 or largest < req.n:
                 return admitted
-            sub, geom = self.subs.acquire(req.key, req.make_geometry)
-            g = torch.from_numpy(np.ascontiguousarray(req.g, np.float32)).to(self.device)
-            ga = torch.from_numpy(np.ascontiguousarray(req.garea, np.float32)).to(self.device)
-            dirs = torch.from_numpy(np.ascontiguousarray(req.dirs, np.float32)).to(self.device)
-            sc = torch.from_numpy(np.ascontiguousarray(req.scales, np.float32)).to(self.device)
-            targets = req.targets if req.targets is not None else np.arange(geom.nstructures + 1)
-            k = self.subs.params(req.key)["k"] or self.k
+            try:  # a substrate that cannot be built fails its request only
+                sub, geom = self.subs.acquire(req.key, req.make_geometry)
+            except Exception as e:
+                self._answer_error(req, e)
+                continue
             try:
+                g = torch.from_numpy(np.ascontiguousarray(req.g, np.float32)).to(self.device)
+                ga = torch.from_numpy(np.ascontiguousarray(req.garea, np.float32)).to(self.device)
+                dirs = torch.from_numpy(np.ascontiguousarray(req.dirs, np.float32)).to(self.device)
+                sc = torch.from_numpy(np.ascontiguousarray(req.scales, np.float32)).to(self.device)
+                targets = req.targets if req.targets is not None else np.arange(geom.nstructures + 1)
+                k = self.subs.params(req.key)["k"] or self.k
                 sim = self.batch.add(sub, req.n, req.seed, req.nsteps, req.dt, g=g, garea=ga,
                                      k=k, targets=targets)
             except BatchFull:
@@ -219,11 +231,7 @@ class Server:
                 return admitted
             except Exception as e:  # the request cannot run: answer it, keep serving
                 self.subs.release(req.key)
-                with self.lock:
-                    self.queue.popleft()
-                now = time.perf_counter()
-                self.results[req.rid] = Result(req.rid, None, req.arrival, now, now,
-                                               error=f"{type(e).__name__}: {e}")
+                self._answer_error(req, e)
                 continue
             with self.lock:
                 self.queue.popleft()

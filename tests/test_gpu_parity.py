@@ -228,6 +228,32 @@ def test_phase_signal_parity(c0_geometry, oracle_lib):
     assert np.abs(ph_gpu - ph_orc).max() < 1e-3
 
 
+def test_projection_of_strided_directions():
+    """Directions handed as a transposed (strided) view give the same phases
+    and signal sums as their dense copy: the dense copy the wrapper makes is
+    the one whose address the kernel reads, held until the launch is queued
+    (interleaved allocations on the same stream cannot take its memory)."""
+    import torch
+    from nozomi_amd import runtime as rt
+    gen = torch.Generator().manual_seed(3)
+    n, W = 50000, 96
+    mom = torch.randn((3, n), generator=gen).to("cuda:0")
+    dirs_wt = torch.randn((W, 3), generator=gen).to("cuda:0")
+    scale = torch.rand(W, generator=gen).to("cuda:0")
+    view = dirs_wt.T  # [3, W], not contiguous
+    assert not view.is_contiguous()
+    dense = view.contiguous()
+    ref_s = rt.signal_sums(mom, dense, scale).cpu().numpy()
+    ref_p = rt.project_phase(mom, dense, scale).cpu().numpy()
+    for _ in range(3):  # allocations between the wrapper's copy and the launch
+        got_s = rt.signal_sums(mom, view, scale)
+        junk = torch.full((3 * W,), 7.0, device="cuda:0")
+        got_p = rt.project_phase(mom, view, scale)
+        junk.fill_(-3.0)
+        assert np.array_equal(got_s.cpu().numpy(), ref_s)
+        assert np.array_equal(got_p.cpu().numpy(), ref_p)
+
+
 def test_free_diffusion_signal():
     """Empty periodic box: S = exp(-b D) (closed form) for every direction."""
     from nozomi_amd.simulation_engine.geometry import SimGeometry3D

@@ -37,6 +37,33 @@ def test_served_results_equal_alone():
     _compare(got, alone)
 
 
+def test_failing_request_is_answered_and_serving_continues():
+    """A request whose substrate cannot be built (its geometry callback
+    raises) and one that cannot be seeded are answered with an error; the
+    requests around them still run and equal their results alone."""
+    import dataclasses
+    reqs, _ = serving_mix(rate=40.0, duration=0.3, seed=7, scale_steps=0.05, scale_spins=0.05)
+    reqs = reqs[:4]
+
+    def broken_geometry():
+        raise OSError("substrate file unreadable")
+
+    bad_sub = dataclasses.replace(reqs[1], rid=1001, key=("broken", 1),
+                                  make_geometry=broken_geometry)
+    bad_seed = dataclasses.replace(reqs[2], rid=1002, targets=np.array([10**6]))
+    cap = sum(r.n for r in reqs) + 2048
+    srv = Server(cap, max_sims=8, k=32, device=DEV)
+    got, _ = serve_trace(srv, [reqs[0], bad_sub, reqs[1], bad_seed, reqs[2], reqs[3]],
+                         [0.0] * 6)
+    srv.close()
+    assert got[1001].signals is None and "substrate file unreadable" in got[1001].error
+    assert got[1002].signals is None and got[1002].error
+    good = {r.rid: got[r.rid] for r in reqs}
+    assert all(v.error is None for v in good.values())
+    alone, _ = run_serial(reqs, device=DEV)
+    _compare(good, alone)
+
+
 def test_batched_mix_equals_alone():
     reqs = batching_mix(nsims=12, seed=9, scale_steps=0.03, scale_spins=0.02)
     got, secs = run_batched(reqs, capacity=sum(r.n for r in reqs) + 256 * len(reqs), k=32,
