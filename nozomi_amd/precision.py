@@ -101,6 +101,10 @@ class PrecisionRun:
         self.signals_out = torch.empty((self.W, 2), dtype=torch.float64, device=d)
         self.walk_stream = torch.cuda.current_stream(d)
         self.gemm_stream = torch.cuda.Stream(d) if overlap else self.walk_stream
+        if overlap:  # memory the contraction stream reads or writes outlives its queued work
+            for t in ([self.phase, self.g, self.u, self.gsum] + self.rec + self.slot_spin +
+                      self.bt_hi + self.bt_lo + [b for b in self.bt_f32 if b is not None]):
+                t.record_stream(self.gemm_stream)
         self.walk_done = [torch.cuda.Event() for _ in range(nb)]
         self.gemm_done = [torch.cuda.Event() for _ in range(nb)]
         self._used = [False] * nb

@@ -67,10 +67,11 @@ class DiffSim3d:
         self.issetup = False
 
     def set_block_grid(self, nspins):
-        """Launch shape is chosen by the library (one spin per lane, 256-lane
-        workgroups); kept for API compatibility (diffsim3d.py:176-178)."""
-        self.nblock = 256
-        self.ngrid = (int(nspins) + 255) // 256
+        """Launch shape is chosen by the library (one spin per lane, 64-lane
+        workgroups for the walk); kept for API compatibility
+        (diffsim3d.py:176-178)."""
+        self.nblock = 64
+        self.ngrid = (int(nspins) + 63) // 64
 
     def set_segments(self, nsegx=1, nsegy=1, nsegz=1):
         """The reference builds padded per-segment sphere lists here
@@ -229,8 +230,8 @@ class DiffSim3d:
                   Kx4_array, Ky4_array, Kz4_array)
         for a, v in zip(arrays, vals):
             t = a.tensor if isinstance(a, DeviceArray) else a
-            if torch.is_tensor(t):
-                t[step_idx] += v.to(t.dtype)
+            if torch.is_tensor(t):  # on any device (a host tensor takes the value over)
+                t[step_idx] += v.to(t.device, t.dtype)
             else:
                 t[step_idx] += float(v)
 
