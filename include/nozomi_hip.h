@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define NZ_ABI_VERSION 4
+#define NZ_ABI_VERSION 5
 
 typedef struct nz_substrate* nz_substrate_t;
 
@@ -248,6 +248,23 @@ int nz_batch_signals(nz_batch_t b, int32_t sim, int32_t W, const float* dirs, co
  * record. */
 int nz_walk_record(nz_walker_t w, uint64_t seed, int32_t nsteps, double dt, float* rec,
                    int64_t ld, int32_t* slot_spin, void* stream);
+/* nz_walk_record into a ring of `ring` steps per slot (ld >= 3 ring): step
+ * t of the block goes to entries 3 e + a, e = (t - t0 + pos0) mod ring, the
+ * block itself to entries pos0 .. pos0 + nsteps - 1 (pos0 + nsteps <= ring),
+ * so a contraction reads it at rec + 3 pos0 with the same ld.  A spin whose
+ * block is done walks on, up to ahead further steps (nsteps + ahead <= ring),
+ * while the other spins of its wavefront finish theirs, recording those steps
+ * at the following ring entries; the tail of the wavefront's block then walks
+ * the next block instead of idling.  The next record call continues in the
+ * same ring at entry (pos0 + nsteps) mod ring and is at least ahead steps
+ * long.  No re-sort of the spins runs while steps run ahead sit in the ring:
+ * the block before a due re-sort runs no steps ahead.  The caller keeps the
+ * entries a contraction still reads out of the next ahead entries (with the
+ * contraction of block b beside the walk of block b + 1: ring >= 3 blocks).
+ * A plain nz_walk, or new positions or steps, end the run-ahead. */
+int nz_walk_record_ring(nz_walker_t w, uint64_t seed, int32_t nsteps, int32_t ahead, double dt,
+                        float* rec, int64_t ld, int32_t ring, int32_t pos0, int32_t* slot_spin,
+                        void* stream);
 /* Waveform block for steps t0..t0+K-1: g [W][T] per-step coefficients,
  * u [3][W] directions; Bt as BF16 hi/lo [W][3K] (uint16 bits) and, if
  * bt_f32 is not NULL, FP32. */

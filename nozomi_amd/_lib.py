@@ -9,7 +9,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libnozomi_hip.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 NZ_WALK_T2 = 1
 NZ_WALKER_COUNTERS = 1
@@ -72,6 +72,9 @@ _SIGS = {
     "nz_walker_time_kernels": (ctypes.c_int, [_c_vp, ctypes.c_int32]),
     "nz_walk_record": (ctypes.c_int, [_c_vp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double, _c_vp,
                                       ctypes.c_int64, _c_vp, _c_vp]),
+    "nz_walk_record_ring": (ctypes.c_int, [_c_vp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_double, _c_vp, ctypes.c_int64, ctypes.c_int32,
+                                           ctypes.c_int32, _c_vp, _c_vp]),
     "nz_waveform_block": (ctypes.c_int, [_c_vp, ctypes.c_int64, _c_vp, ctypes.c_int32, ctypes.c_int64,
                                          ctypes.c_int32, _c_vp, _c_vp, _c_vp, _c_vp]),
     "nz_phase_gemm": (ctypes.c_int, [_c_vp, ctypes.c_int64, ctypes.c_int64, _c_vp, ctypes.c_int32,

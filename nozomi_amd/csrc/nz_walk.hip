@@ -47,11 +47,19 @@ struct Walker {
   // instead of waiting for the three others of a 256-thread workgroup (200-axon
   // substrate, 1e6 spins, 250-step launches: 13.37 ms against 13.95 ms).
   int32_t walk_block = 64;
-  // threads per moments_kernel / record_kernel workgroup (NZ_TRACK_BLOCK); 256 measured faster
-  // here than 64 (the per-step moment partials are per workgroup)
+  // threads per moments_kernel workgroup (NZ_TRACK_BLOCK); 256 measured faster here than 64
+  // (the per-step moment partials are per workgroup)
   int32_t track_block = 256;
   int32_t since_sort = 1 << 30;
   int64_t step = 0;
+  // the last record call let spins run up to rec_ahead steps past its block
+  // (nz_walk_record_ring); those steps sit in its ring, which the next record
+  // block continues at entry ring_next
+  int32_t rec_ahead = 0;
+  int32_t ring_next = 0;
+  int32_t ring = 0;
+  int64_t rec_ld = 0;
+  const float* rec_ptr = nullptr;
   std::vector<void*> allocations;
   int device = 0;
   // walk_kernel launch timing (nz_walker_time_kernels): one start/stop event
@@ -77,6 +85,8 @@ struct WalkArgs {
   int32_t xcd_chunk;  // slot blocks per XCD chunk (0: no remap)
   float* rec;         // displacement record (nz_walk_record) or null
   int64_t rec_ld;     // floats per slot of the record
+  int32_t rec_ring = 0;  // record entries per slot, used as a ring of steps
+  int32_t rec_pos0 = 0;  // ring entry of the launch's first step (mom_base)
 };
 
 // Workgroups of a launch are dispatched to the 8 XCDs round-robin (block b
@@ -149,7 +159,9 @@ struct StepParams {
   double* stepmom;   // per-step moment sums (needs smom and target - mom_base <= kMomChunk)
   uint32_t flags;
   float* rec;        // per-step displacement record [slots][rec_ld] or null
-  int64_t rec_ld;    // floats per slot of the record (>= 3 * (target - mom_base))
+  int64_t rec_ld;    // floats per slot of the record (>= 3 * rec_ring)
+  uint32_t rec_ring = 0;  // step t goes to ring entry (t - mom_base + rec_pos0) mod rec_ring
+  uint32_t rec_pos0 = 0;
 };
 
 // What a walk launch keeps beside the walk itself.  The tracking kinds hold
@@ -223,7 +235,9 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
         if (kKind == kRecord) {  // displacement since the start, for the blocked phase contraction
           // slot-major: a lane's steps are contiguous (the lanes of a
           // wavefront are at different steps, so rows of steps would scatter)
-          float* r = P.rec + i * P.rec_ld + (int64_t)(t - P.mom_base) * 3;
+          uint32_t e = t - P.mom_base + P.rec_pos0;  // < 2 rec_ring
+          if (e >= P.rec_ring) e -= P.rec_ring;
+          float* r = P.rec + i * P.rec_ld + (int64_t)e * 3;
           r[0] = (float)lane_disp(g, s, x0, 0);
           r[1] = (float)lane_disp(g, s, x0, 1);
           r[2] = (float)lane_disp(g, s, x0, 2);
@@ -298,7 +312,7 @@ __global__ void __launch_bounds__(256) NZ_TRACK_ATTR record_kernel(const WalkArg
   const int64_t i = blk * blockDim.x + threadIdx.x;
   const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
                      (uint32_t)A.mom_base, A.dt, A.gph, A.garea, nullptr, A.flags, A.rec,
-                     A.rec_ld};
+                     A.rec_ld, (uint32_t)A.rec_ring, (uint32_t)A.rec_pos0};
   walk_slot<kRecord>(*gpr, A.st, A.n, i, i < A.n, P, nullptr);
 }
 
@@ -1124,6 +1138,7 @@ int nz_walker_set_step(nz_walker_t ww, int64_t step) {
   if (int rc = check_walker(w, "nz_walker_set_step")) return rc;
   if (step < 0 || step > (int64_t)UINT32_MAX) return fail(-1, "nz_walker_set_step: out of range");
   w->step = step;
+  w->rec_ahead = 0;
   hipLaunchKernelGGL(fill_u32, dim3(blocks_for(w->n)), dim3(256), 0, (hipStream_t)0,
                      w->buf[w->cur].tstep, w->n, (uint32_t)step);
   NZ_HIP(hipGetLastError());
@@ -1141,6 +1156,7 @@ int nz_walker_set_positions(nz_walker_t ww, const float* pos, void* stream) {
   NZ_HIP(hipGetLastError());
   w->step = 0;
   w->since_sort = 1 << 30;
+  w->rec_ahead = 0;
   return 0;
 }
 
@@ -1204,21 +1220,45 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
     w->since_sort += (int32_t)chunk;
   }
   w->step = end;
+  w->rec_ahead = 0;  // steps walked past a record block are not recorded by this walk
   return 0;
 }
 
-int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, float* rec,
-                   int64_t ld, int32_t* slot_spin, void* stream) {
+int nz_walk_record_ring(nz_walker_t ww, uint64_t seed, int32_t nsteps, int32_t ahead, double dt,
+                        float* rec, int64_t ld, int32_t ring, int32_t pos0, int32_t* slot_spin,
+                        void* stream) {
   Walker* w = (Walker*)ww;
   if (int rc = check_walker(w, "nz_walk_record")) return rc;
   if (nsteps <= 0 || nsteps > 4096) return fail(-1, "nz_walk_record: nsteps must be in [1, 4096]");
+  if (ahead < 0) return fail(-1, "nz_walk_record: ahead must be >= 0");
   if (!(dt >= 0.0)) return fail(-1, "nz_walk_record: dt must be >= 0");
   if (!rec || !slot_spin) return fail(-1, "nz_walk_record: NULL rec or slot_spin");
-  if (ld < 3 * (int64_t)nsteps) return fail(-1, "nz_walk_record: ld must be at least 3 * nsteps");
-  if (w->step + nsteps > (int64_t)UINT32_MAX) return fail(-1, "nz_walk_record: step counter overflow");
+  if (ring < 1 || ld < 3 * (int64_t)ring)
+    return fail(-1, "nz_walk_record: ld must be at least 3 * ring, ring >= 1");
+  if (pos0 < 0 || (int64_t)pos0 + nsteps > ring)
+    return fail(-1, "nz_walk_record: the block must fit the ring from pos0 without wrapping");
+  if ((int64_t)nsteps + ahead > ring)
+    return fail(-1, "nz_walk_record: nsteps + ahead must not exceed the ring");
+  if (w->step + nsteps + ahead > (int64_t)UINT32_MAX)
+    return fail(-1, "nz_walk_record: step counter overflow");
+  if (w->rec_ahead > 0 && (rec != w->rec_ptr || ld != w->rec_ld || ring != w->ring ||
+                           pos0 != w->ring_next))
+    return fail(-1, "nz_walk_record: the previous record block ran ahead into its ring: "
+                    "continue in the same ring (rec, ld, ring) at entry " +
+                        std::to_string(w->ring_next));
+  if (w->rec_ahead > nsteps)
+    return fail(-1, "nz_walk_record: the previous record block ran further ahead than this "
+                    "block is long");
   hipStream_t s = (hipStream_t)stream;
-  if (w->sort_interval > 0 && w->since_sort >= w->sort_interval)
+  // spins stand at different steps after a block that ran ahead, and their
+  // steps past it sit in its slots: no re-sort until a block ends with every
+  // spin at its end
+  const bool sort_due = w->sort_interval > 0 && w->since_sort >= w->sort_interval;
+  if (sort_due && w->rec_ahead == 0)
     if (int rc = sort_spins(w, s)) return rc;
+  // the block before a re-sort does not run ahead
+  const bool sort_next = w->sort_interval > 0 && w->since_sort + nsteps >= w->sort_interval;
+  if (sort_next) ahead = 0;
   WalkArgs a;
   a.st = w->buf[w->cur];
   a.n = w->n;
@@ -1226,7 +1266,9 @@ int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, flo
   a.base = w->step;
   a.mom_base = w->step;
   a.target = w->step + nsteps;
-  a.limit = a.target;  // every spin stops at the block end: the record is complete
+  // a spin whose block is done walks on, up to ahead steps, while others of
+  // its wavefront finish theirs; the record is complete up to the block end
+  a.limit = a.target + ahead;
   a.dt = dt;
   a.gph = nullptr;
   a.garea = nullptr;
@@ -1235,14 +1277,34 @@ int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, flo
   a.xcd_chunk = w->xcd_chunk;
   a.rec = rec;
   a.rec_ld = ld;
-  hipLaunchKernelGGL(record_kernel, dim3(blocks_for(w->n, w->track_block)), dim3(w->track_block), 0, s, a,
-                     (const Grid*)w->sub->g_dev);
+  a.rec_ring = ring;
+  a.rec_pos0 = pos0;
+  // one wave per workgroup, as the plain walk (precision form, 4e6 spins,
+  // 1280 steps at K = 64: 0.531 s against 0.537 s in 256-thread workgroups)
+  hipLaunchKernelGGL(record_kernel, dim3(blocks_for(w->n, w->walk_block)), dim3(w->walk_block), 0,
+                     s, a, (const Grid*)w->sub->g_dev);
   NZ_HIP(hipGetLastError());
   NZ_HIP(hipMemcpyAsync(slot_spin, w->buf[w->cur].spin_id, sizeof(int32_t) * w->n,
                         hipMemcpyDeviceToDevice, s));
   w->step += nsteps;
   w->since_sort += nsteps;
+  w->rec_ahead = ahead;
+  w->ring_next = (pos0 + nsteps) % ring;
+  w->ring = ring;
+  w->rec_ld = ld;
+  w->rec_ptr = rec;
   return 0;
+}
+
+int nz_walk_record(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, float* rec,
+                   int64_t ld, int32_t* slot_spin, void* stream) {
+  if (ww && ((Walker*)ww)->rec_ahead > 0)
+    return fail(-1, "nz_walk_record: the previous record block ran ahead: continue it with "
+                    "nz_walk_record_ring");
+  if (ld < 3 * (int64_t)nsteps || ld / 3 > INT32_MAX)
+    return fail(-1, "nz_walk_record: ld must be at least 3 * nsteps");
+  return nz_walk_record_ring(ww, seed, nsteps, 0, dt, rec, ld, (int32_t)(ld / 3), 0, slot_spin,
+                             stream);
 }
 
 int nz_walker_read(nz_walker_t ww, float* pos, float* pos0, float* sig, float* mom, void* stream) {

@@ -182,3 +182,85 @@ def test_precision_form_full_size_two_blocks():
           f"{(err / (scale + 1e-30)).max():.2e}", flush=True)
     assert (err / (scale + 1e-30)).max() < 6e-5
     assert err.max() < 2e-3
+
+
+def test_run_ahead_record_equals_stopping_at_block_ends():
+    """Spins whose block is done walk up to `ahead` steps of the next block
+    while the slower spins of their wavefront finish (nz_walk_record_ring),
+    leaving those steps in the ring for the next block.  Every block's record
+    (in spin order) and the phases equal those of the run whose spins all stop
+    at each block end, bit for bit: with and without the contraction on its own
+    stream, re-sorting every block or every fourth, and with a short last
+    block."""
+    M, L = generate_beaded_axons(16, 30.0, seed=20261203)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    sub = geom.device_substrate(device=DEV)
+    N, T, dt, W, K, seed = 20000, 520, 0.02, 64, 64, 31  # eight blocks of 64 steps and one of 8
+    g, u, _ = precision_waveforms(W, T, dt, seed=9)
+    x0 = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=6)
+    out = {}
+    for ahead, overlap, interval in ((0, False, 64), (64, False, 256), (64, True, 256),
+                                     (32, True, 128), (64, True, 64)):
+        recs = []
+
+        def hook(t0, k, rec, slot_spin):
+            r = torch.empty((N, 3 * k), dtype=torch.float32, device=DEV)
+            r[slot_spin.long()] = rec
+            recs.append(r.cpu().numpy())
+
+        pr = PrecisionRun(sub, x0, seed, g, u, dt, K=K, mode="mfma", overlap=overlap, ahead=ahead,
+                          sort_interval=interval)
+        pr.run(record_hook=hook)
+        pr.finish()
+        out[(ahead, overlap, interval)] = (recs, pr.phase.cpu().numpy().copy(), pr.signals())
+        pr.close()
+    ref = out[(0, False, 64)]
+    assert len(ref[0]) == 9
+    for key, (recs, ph, sig) in out.items():
+        print(f"ahead {key[0]}, overlap {key[1]}, re-sort every {key[2]} steps: "
+              "records and phases equal", flush=True)
+        assert len(recs) == len(ref[0])
+        for a, b in zip(recs, ref[0]):
+            assert np.array_equal(a, b), key
+        assert np.array_equal(ph, ref[1]), key
+        assert np.array_equal(sig, ref[2]), key
+
+
+def test_run_ahead_record_argument_rules():
+    """nz_walk_record_ring refuses, before any launch: a block that wraps the
+    ring, block + ahead longer than the ring, rows shorter than the ring, a
+    record call after a run-ahead block that does not continue its ring where
+    it ended, and a block shorter than the steps run ahead."""
+    from nozomi_amd import runtime as rt
+    M, L = generate_beaded_axons(6, 20.0, seed=20261203)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    sub = geom.device_substrate(device=DEV)
+    N = 1024
+    w = rt.Walker(sub, N, sort_interval=1024)
+    w.set_positions(sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=1))
+    rec = torch.zeros((N, 3 * 128), dtype=torch.float32, device=DEV)
+    other = torch.zeros_like(rec)
+    ss = torch.empty(N, dtype=torch.int32, device=DEV)
+    Lb, st = lib(), _stream_ptr(DEV)
+
+    def call(nsteps, ahead, pos0, r=rec, ld=3 * 128, ring=128):
+        return Lb.nz_walk_record_ring(w.handle, 5, nsteps, ahead, 0.02, _ptr(r), ld, ring, pos0,
+                                      _ptr(ss), st)
+
+    def err():
+        return Lb.nz_last_error_string()
+
+    assert call(64, 0, 96) < 0 and b"wrapping" in err()
+    assert call(64, 65, 0) < 0 and b"exceed the ring" in err()
+    assert call(64, 0, 0, ld=3 * 127) < 0 and b"ld" in err()
+    assert call(64, 32, 0) == 0                   # runs up to 32 steps ahead
+    assert call(64, 0, 0) < 0 and b"entry 64" in err()
+    assert call(64, 0, 64, r=other) < 0 and b"same ring" in err()
+    assert call(64, 0, 64, ld=3 * 130) < 0 and b"same ring" in err()
+    assert Lb.nz_walk_record(w.handle, 5, 64, 0.02, _ptr(rec), 3 * 128, _ptr(ss), st) < 0
+    assert call(16, 0, 64) < 0 and b"further ahead" in err()
+    assert call(64, 32, 64) == 0                  # continues at entry 64, ahead into entry 0..
+    assert call(32, 0, 0) == 0                    # takes those steps over and stops at its end
+    assert Lb.nz_walk_record(w.handle, 5, 64, 0.02, _ptr(rec), 3 * 128, _ptr(ss), st) == 0
+    torch.cuda.synchronize()
+    w.close()
