@@ -89,15 +89,18 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // + v % 4 for v = 0..15.
 constexpr int kTileS = 128;  // slots per workgroup
 constexpr int kTileW = 32;   // waveforms per tile
-constexpr int kChunk = 32;   // k per chunk interval (two 16-k MFMA steps)
-constexpr int kRing = 6;     // waveform chunk entries in LDS
 
-// Position of k (a multiple of 8) in LDS row `col` of a waveform chunk: rows
-// of 32 BF16 (64 bytes, four 16-byte pieces), piece index XORed with bits 2-3
-// of the row, so the 32 rows x 2 pieces of an MFMA B read use every bank
-// pair-wise (the minimum for 1 KB) and the ring stores (16 rows x 4 pieces)
+// Position of k (a multiple of 8) in LDS row `col` of a waveform chunk of CH
+// k (32 or 64): rows of CH BF16 (CH / 8 16-byte pieces), the piece index
+// XORed with bits 2-3 of the row (CH = 32) or with bits 0-2 of the row and
+// bit 3 (CH = 64), so the rows of any 16 consecutive lanes of an MFMA B read
+// fall on 16 different 16-byte bank groups and the 1 KB read uses every bank
+// the minimum number of times; the ring stores (1 KB of whole rows each)
 // likewise.
-__device__ __forceinline__ int bpiece(int col, int k) { return 8 * ((k >> 3) ^ ((col >> 2) & 3)); }
+template <int CH>
+__device__ __forceinline__ int bswz(int col) {
+  return CH == 32 ? ((col >> 2) & 3) : ((col & 7) ^ ((col >> 3) & 1));
+}
 // Word of (row r, column c) of a 128 x 32 FP32 phase tile in LDS: rows of 32
 // words, the 4-word group index XORed with 4 ((r / 4) % 2).  Phase-wave load
 // j (0..15) of lane l covers words 256 j + 4 l .. + 4: row 8 j + l / 8,
@@ -133,15 +136,22 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
 // LDS writes done, then the workgroup barrier.
 __device__ __forceinline__ void wg_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// NKC chunk intervals per tile, KS parts of K: K3 = 32 NKC KS.
-template <int NKC, int KS>
+// NKC chunk intervals of CH k per tile, KS parts of K: K3 = CH NKC KS.  A
+// chunk interval ends with a workgroup barrier: 64-k chunks halve the
+// barriers per tile (the MFMA waves read a chunk's B operands in two halves
+// into the same registers), with a ring of four entries instead of six.
+template <int NKC, int KS, int CH>
 __global__ void __launch_bounds__(64 * (4 * KS + 2))
 __attribute__((amdgpu_waves_per_eu(KS == 3 ? 4 : 3)))
 phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin,
                        const uint16_t* bt_hi, const uint16_t* bt_lo, int32_t W, float* phase) {
+  constexpr int kChunk = CH;
+  static_assert(CH == 32 || CH == 64, "chunks of 32 or 64 k");
+  constexpr int kRing = CH == 32 ? 6 : 4;  // waveform chunk entries in LDS
   constexpr int KP = NKC * kChunk;  // k per part
   constexpr int K3 = KP * KS;
   constexpr int kEntry = KS * 2 * kTileW * kChunk;  // uint16 per ring entry
+  constexpr int kEntryLoads = KS * 2 * (kChunk / 16);  // 1 KB loads per ring entry
   constexpr int kTileWords = kTileS * kTileW;
   __shared__ __attribute__((aligned(16))) uint16_t sB[kRing * kEntry];
   __shared__ __attribute__((aligned(16))) float sP[2 * kTileWords];
@@ -153,13 +163,13 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
   if (wave < 4 * KS) {  // ---- MFMA wave (slot group sg, part kq)
     const int sg = wave & 3, kq = wave >> 2;
     const int kh = 8 * (lane >> 5), col = lane & 31;
-    bf16x8 ah[2 * NKC], al[2 * NKC];
+    bf16x8 ah[NKC * (kChunk / 16)], al[NKC * (kChunk / 16)];
     {
       const int64_t srow = s0 + sg * 32 + col;
       const bool row_ok = srow < nslots;
       const float* arow = rec + (row_ok ? srow : 0) * ld + kq * KP + kh;
 #pragma unroll
-      for (int ks = 0; ks < 2 * NKC; ++ks) {
+      for (int ks = 0; ks < NKC * (kChunk / 16); ++ks) {
         floatx4 v[2];
         __builtin_memcpy(&v[0], arow + ks * 16, 16);
         __builtin_memcpy(&v[1], arow + ks * 16 + 4, 16);
@@ -184,7 +194,9 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
 #pragma unroll
     for (int j = 0; j < 4; ++j) psp[j] = __shfl(spin_row, 8 * j + (lane >> 3), 64);
     const int pcol = ppiece_col(lane);
-    const int bo0 = col * kChunk + bpiece(col, kh), bo1 = col * kChunk + bpiece(col, 16 + kh);
+    int bo[kChunk / 16];  // the lane's B piece of 16-k step s: k = 16 s + kh of row col
+#pragma unroll
+    for (int s = 0; s < kChunk / 16; ++s) bo[s] = col * kChunk + 8 * ((2 * s + (kh >> 3)) ^ bswz<CH>(col));
     wg_barrier();
     int g = 0;
     for (int t = 0; t < ntiles; ++t) {
@@ -195,17 +207,21 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
       for (int kc = 0; kc < NKC; ++kc, ++g) {
         const uint16_t* bh = sB + (g % kRing) * kEntry + kq * (2 * kTileW * kChunk);
         const uint16_t* bl = bh + kTileW * kChunk;
-        bf16x8 h0, l0, h1, l1;
-        __builtin_memcpy(&h0, bh + bo0, 16);
-        __builtin_memcpy(&l0, bl + bo0, 16);
-        __builtin_memcpy(&h1, bh + bo1, 16);
-        __builtin_memcpy(&l1, bl + bo1, 16);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[2 * kc], h0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[2 * kc], l0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[2 * kc], h0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[2 * kc + 1], h1, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[2 * kc + 1], l1, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[2 * kc + 1], h1, acc, 0, 0, 0);
+#pragma unroll
+        for (int s2 = 0; s2 < kChunk / 32; ++s2) {  // two 16-k steps per half
+          const int a0 = kc * (kChunk / 16) + 2 * s2;
+          bf16x8 h0, l0, h1, l1;
+          __builtin_memcpy(&h0, bh + bo[2 * s2], 16);
+          __builtin_memcpy(&l0, bl + bo[2 * s2], 16);
+          __builtin_memcpy(&h1, bh + bo[2 * s2 + 1], 16);
+          __builtin_memcpy(&l1, bl + bo[2 * s2 + 1], 16);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a0], h0, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a0], l0, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a0], h0, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a0 + 1], h1, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a0 + 1], l1, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a0 + 1], h1, acc, 0, 0, 0);
+        }
         wg_barrier();
       }
       float* tp = sP + (t & 1) * kTileWords;
@@ -238,19 +254,21 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
 #pragma unroll
         for (int hl = 0; hl < 2; ++hl)
 #pragma unroll
-          for (int cg = 0; cg < 2; ++cg) {
-            const int c = cg * 16 + (lane >> 2);
-            const int kp = (lane & 3) ^ ((c >> 2) & 3);  // LDS piece lane & 3 holds k-piece kp
+          for (int cg = 0; cg < kChunk / 16; ++cg) {  // 1 KB (64 pieces) per load
+            constexpr int kPieces = kChunk / 8;          // 16-byte pieces per row
+            const int e = cg * 64 + lane, c = e / kPieces;
+            // LDS piece e % kPieces of row c holds k-piece kp
+            const int kp = (e - c * kPieces) ^ bswz<CH>(c);
             const int w = t * kTileW + c;
             const uint16_t* src = (hl ? bt_lo : bt_hi) + (int64_t)(w < W ? w : 0) * K3 + q * KP +
                                   kc * kChunk + 8 * kp;
-            __builtin_amdgcn_global_load_lds(src, dst + (q * 2 + hl) * (kTileW * kChunk) + cg * 16 * kChunk,
+            __builtin_amdgcn_global_load_lds(src, dst + (q * 2 + hl) * (kTileW * kChunk) + cg * 512,
                                              16, 0, 0);
           }
     };
     const int first = nint < kRing ? nint : kRing;
     for (int gi = 0; gi < first; ++gi) issue(gi);
-    wait_vmcnt(4 * KS * (first - 1));  // entry 0 landed
+    wait_vmcnt(kEntryLoads * (first - 1));  // entry 0 landed
     wg_barrier();
     int g = 0;
     for (int t = 0; t < ntiles; ++t) {
@@ -259,7 +277,7 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
         if (g >= 1 && g + kRing - 1 < nint) issue(g + kRing - 1);
         if (g + 1 < nint) {  // entry g + 1 lands before the barrier that opens it
           const int last = (g + kRing - 1 < nint) ? g + kRing - 1 : nint - 1;
-          wait_vmcnt(4 * KS * (last - (g + 1)));
+          wait_vmcnt(kEntryLoads * (last - (g + 1)));
         }
         wg_barrier();
       }
@@ -297,26 +315,28 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
   }
 }
 
+constexpr int kF32Chunk = 32;  // k per waveform chunk of the FP32 kernel
+
 // The same contraction on the vector ALUs in FP32 (the CUDA-core form).
 // Thread: one slot, 16 waveforms; the block's waveform columns via LDS.
 __global__ void __launch_bounds__(256) phase_gemm_fp32_kernel(
     const float* rec, int64_t ld, int64_t nslots, const int32_t* slot_spin, int32_t K3,
     const float* bt, int32_t W, float* phase) {
-  __shared__ float sB[16][kChunk];
+  __shared__ float sB[16][kF32Chunk];
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int w0 = blockIdx.y * 16;
   float acc[16];
 #pragma unroll
   for (int c = 0; c < 16; ++c) acc[c] = 0.0f;
-  for (int k0 = 0; k0 < K3; k0 += kChunk) {
-    for (int e = threadIdx.x; e < 16 * kChunk; e += blockDim.x) {
-      const int ww = e / kChunk, kk = e % kChunk;
+  for (int k0 = 0; k0 < K3; k0 += kF32Chunk) {
+    for (int e = threadIdx.x; e < 16 * kF32Chunk; e += blockDim.x) {
+      const int ww = e / kF32Chunk, kk = e % kF32Chunk;
       const int w = w0 + ww;
       sB[ww][kk] = (w < W && k0 + kk < K3) ? bt[(int64_t)w * K3 + k0 + kk] : 0.0f;
     }
     __syncthreads();
     if (s < nslots) {
-      for (int kk = 0; kk < kChunk && k0 + kk < K3; ++kk) {
+      for (int kk = 0; kk < kF32Chunk && k0 + kk < K3; ++kk) {
         const float x = rec[s * ld + k0 + kk];
 #pragma unroll
         for (int c = 0; c < 16; ++c) acc[c] = fmaf(x, sB[c][kk], acc[c]);
@@ -416,16 +436,16 @@ int nz_phase_gemm(const float* rec, int64_t ld, int64_t nslots, const int32_t* s
     dim3 grid((unsigned)((nslots + kTileS - 1) / kTileS));
     // K = 32, 64: one part; 96: three parts of 32 steps; 128: two parts of 64
     if (K3 == 96)
-      hipLaunchKernelGGL((phase_gemm_mfma_kernel<3, 1>), grid, dim3(64 * 6), 0, (hipStream_t)stream,
+      hipLaunchKernelGGL((phase_gemm_mfma_kernel<3, 1, 32>), grid, dim3(64 * 6), 0, (hipStream_t)stream,
                          rec, ld, nslots, slot_spin, bt_hi, bt_lo, W, phase);
     else if (K3 == 192)
-      hipLaunchKernelGGL((phase_gemm_mfma_kernel<6, 1>), grid, dim3(64 * 6), 0, (hipStream_t)stream,
+      hipLaunchKernelGGL((phase_gemm_mfma_kernel<3, 1, 64>), grid, dim3(64 * 6), 0, (hipStream_t)stream,
                          rec, ld, nslots, slot_spin, bt_hi, bt_lo, W, phase);
     else if (K3 == 288)
-      hipLaunchKernelGGL((phase_gemm_mfma_kernel<3, 3>), grid, dim3(64 * 14), 0, (hipStream_t)stream,
+      hipLaunchKernelGGL((phase_gemm_mfma_kernel<3, 3, 32>), grid, dim3(64 * 14), 0, (hipStream_t)stream,
                          rec, ld, nslots, slot_spin, bt_hi, bt_lo, W, phase);
     else
-      hipLaunchKernelGGL((phase_gemm_mfma_kernel<6, 2>), grid, dim3(64 * 10), 0, (hipStream_t)stream,
+      hipLaunchKernelGGL((phase_gemm_mfma_kernel<3, 2, 64>), grid, dim3(64 * 10), 0, (hipStream_t)stream,
                          rec, ld, nslots, slot_spin, bt_hi, bt_lo, W, phase);
   } else {
     if (!bt_f32) return fail(-1, "nz_phase_gemm: FP32 mode needs bt_f32");
