@@ -176,8 +176,11 @@ enum WalkKind { kPlain = 0, kMoments = 1, kRecord = 2 };
 template <int kKind>
 __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, int64_t ld,
                                           int64_t i, bool active, const StepParams P,
-                                          double* smom) {
+                                          double* smom, double* xs = nullptr) {
   const ListBufs lb = list_bufs(g);  // every lane is active here
+  // the recording walk keeps no phase moment and no T2 weight (the precision
+  // form takes neither), leaving their registers to the walk
+  constexpr bool kTrack = kKind != kRecord;
   const int64_t ii = active ? i : 0;
   Lane s;
 #pragma unroll
@@ -185,13 +188,13 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
     s.x[a] = active ? st.x[a * ld + ii] : 0.0;
     s.l[a] = active ? st.cell[a * ld + ii] : 1;
     s.w[a] = active ? st.wrap[a * ld + ii] : 0;
-    s.m[a] = (active && P.gph) ? st.mom[a * ld + ii] : 0.0f;
+    s.m[a] = (kTrack && active && P.gph) ? st.mom[a * ld + ii] : 0.0f;
   }
-  s.logsig = active ? st.logsig[ii] : 0.0f;
+  s.logsig = (kTrack && active) ? st.logsig[ii] : 0.0f;
   s.sid = active ? (float)st.sid[ii] : (float)g.nstruct;
   s.nsub = 0;
   s.nguard = 0;
-  const bool do_t2 = (P.flags & NZ_WALK_T2) != 0;
+  const bool do_t2 = kTrack && (P.flags & NZ_WALK_T2) != 0;
   const uint32_t spin = active ? (uint32_t)st.spin_id[ii] : 0u;
   uint32_t t = active ? st.tstep[ii] : P.target;
   double x0[3] = {0.0, 0.0, 0.0};
@@ -199,9 +202,18 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
     for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x) smom[k] = 0.0;
     __syncthreads();
   }
-  if (kKind != kPlain) {
+  if (kKind == kMoments) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) x0[a] = active ? st.x0[a * ld + ii] : 0.0;
+  }
+  // kRecord: the start position and the record row's offset wait in LDS
+  // (xs: 4 doubles per thread, the offset's bits last) and are read at each
+  // step end; held in registers they were spilled to scratch, and each
+  // reload waited behind the wave's outstanding vector loads
+  if (kKind == kRecord) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xs[4 * threadIdx.x + a] = active ? st.x0[a * ld + ii] : 0.0;
+    xs[4 * threadIdx.x + 3] = __longlong_as_double((long long)(ii * P.rec_ld));
   }
 
   bool started = false;
@@ -225,22 +237,28 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
       // the step's gradient samples, loaded before the sub-step so that their
       // latency hides behind it (used only if the sub-step ends the step)
       float G = 0.0f, Ga = 0.0f;
-      if (P.gph) {
+      if (kTrack && P.gph) {
         G = P.gph[t - P.base];
         Ga = P.garea[t - P.base];
       }
       sched_fence();
       if (substep(g, lb, s, q, P.seed, spin, step, P.dt, do_t2)) {
-        step_end(g, s, G, Ga);
+        step_end(g, s, G, Ga, kTrack);
         if (kKind == kRecord) {  // displacement since the start, for the blocked phase contraction
           // slot-major: a lane's steps are contiguous (the lanes of a
           // wavefront are at different steps, so rows of steps would scatter)
           uint32_t e = t - P.mom_base + P.rec_pos0;  // < 2 rec_ring
           if (e >= P.rec_ring) e -= P.rec_ring;
-          float* r = P.rec + i * P.rec_ld + (int64_t)e * 3;
-          r[0] = (float)lane_disp(g, s, x0, 0);
-          r[1] = (float)lane_disp(g, s, x0, 1);
-          r[2] = (float)lane_disp(g, s, x0, 2);
+          // volatile: read here, each step, not kept in registers
+          typedef double f64x2 __attribute__((ext_vector_type(2)));
+          typedef __attribute__((address_space(3))) const volatile f64x2 lds_f64x2;
+          lds_f64x2* xv = (lds_f64x2*)(xs + 4 * threadIdx.x);
+          const f64x2 u0 = xv[0], u1 = xv[1];
+          const double xl[3] = {u0.x, u0.y, u1.x};
+          float* r = P.rec + __double_as_longlong(u1.y) + (int64_t)e * 3;
+          r[0] = (float)lane_disp(g, s, xl, 0);
+          r[1] = (float)lane_disp(g, s, xl, 1);
+          r[2] = (float)lane_disp(g, s, xl, 2);
         }
         if (kKind == kMoments) {  // moment sums (sim3d_kernel.cu:191-237), FP64 throughout
           const int k = (int)(t - P.mom_base) * 6;
@@ -270,9 +288,9 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
     st.x[a * ld + i] = s.x[a];
     st.cell[a * ld + i] = s.l[a];
     st.wrap[a * ld + i] = s.w[a];
-    if (P.gph) st.mom[a * ld + i] = s.m[a];
+    if (kTrack && P.gph) st.mom[a * ld + i] = s.m[a];
   }
-  st.logsig[i] = s.logsig;
+  if (kTrack) st.logsig[i] = s.logsig;
   st.sid[i] = (int32_t)s.sid;
   st.tstep[i] = t;
   if (st.nsub) st.nsub[i] += s.nsub;
@@ -313,7 +331,8 @@ __global__ void __launch_bounds__(256) NZ_TRACK_ATTR record_kernel(const WalkArg
   const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
                      (uint32_t)A.mom_base, A.dt, A.gph, A.garea, nullptr, A.flags, A.rec,
                      A.rec_ld, (uint32_t)A.rec_ring, (uint32_t)A.rec_pos0};
-  walk_slot<kRecord>(*gpr, A.st, A.n, i, i < A.n, P, nullptr);
+  extern __shared__ double rec_xs[];  // 4 per thread: start position, record row offset
+  walk_slot<kRecord>(*gpr, A.st, A.n, i, i < A.n, P, nullptr, rec_xs);
 }
 
 // Absolute start positions (spin order) -> resident state (slot = spin).
@@ -1281,8 +1300,8 @@ int nz_walk_record_ring(nz_walker_t ww, uint64_t seed, int32_t nsteps, int32_t a
   a.rec_pos0 = pos0;
   // one wave per workgroup, as the plain walk (precision form, 4e6 spins,
   // 1280 steps at K = 64: 0.531 s against 0.537 s in 256-thread workgroups)
-  hipLaunchKernelGGL(record_kernel, dim3(blocks_for(w->n, w->walk_block)), dim3(w->walk_block), 0,
-                     s, a, (const Grid*)w->sub->g_dev);
+  hipLaunchKernelGGL(record_kernel, dim3(blocks_for(w->n, w->walk_block)), dim3(w->walk_block),
+                     (size_t)w->walk_block * 4 * sizeof(double), s, a, (const Grid*)w->sub->g_dev);
   NZ_HIP(hipGetLastError());
   NZ_HIP(hipMemcpyAsync(slot_spin, w->buf[w->cur].spin_id, sizeof(int32_t) * w->n,
                         hipMemcpyDeviceToDevice, s));

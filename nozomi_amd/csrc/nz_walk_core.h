@@ -891,21 +891,22 @@ NZ_HD inline bool substep(const Grid& g, const ListBufs& lb, Lane& s, StepState&
 // :880-887), the periodic wrap once per axis with the spins0 shift and the
 // phase correction (:489-521, :683-715, :891-942), then the cell holding the
 // end point.
-NZ_HD inline void step_end(const Grid& g, Lane& s, float G, float Ga) {
+// with_m false: no phase moment is kept (its registers are left to the walk).
+NZ_HD inline void step_end(const Grid& g, Lane& s, float G, float Ga, bool with_m = true) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    s.m[a] = fmaf(G, (float)s.x[a], s.m[a]);
+    if (with_m) s.m[a] = fmaf(G, (float)s.x[a], s.m[a]);
     const double La = g.Ld[a];
     int l = s.l[a];
     if (s.x[a] < -La / 2) {
       s.x[a] += La;
       s.w[a] += 1;
-      s.m[a] = fmaf(g.L[a], Ga, s.m[a]);
+      if (with_m) s.m[a] = fmaf(g.L[a], Ga, s.m[a]);
       l += g.ncell[a];
     } else if (s.x[a] > La / 2) {
       s.x[a] -= La;
       s.w[a] -= 1;
-      s.m[a] = fmaf(-g.L[a], Ga, s.m[a]);
+      if (with_m) s.m[a] = fmaf(-g.L[a], Ga, s.m[a]);
       l -= g.ncell[a];
     }
     // a step rarely leaves the neighbouring cell: two comparisons, the
