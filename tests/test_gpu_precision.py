@@ -226,6 +226,29 @@ def test_run_ahead_record_equals_stopping_at_block_ends():
         assert np.array_equal(sig, ref[2]), key
 
 
+@pytest.mark.parametrize("K", [32, 96, 128])
+def test_run_ahead_other_block_lengths(K):
+    """The run-ahead ring at the other MFMA block lengths (32-k chunks for
+    K = 32 and 96, two parts for K = 128): phases and signals equal stopping
+    every spin at each block end, bit for bit, with a short last block."""
+    M, L = generate_beaded_axons(16, 30.0, seed=20261203)
+    geom = load_geometry(M, L, D=2.0, T2=100.0)
+    sub = geom.device_substrate(device=DEV)
+    N, dt, W, seed = 12000, 0.02, 40, 8
+    T = 4 * K + 12
+    g, u, _ = precision_waveforms(W, T, dt, seed=K)
+    x0 = sub.seed(N, targets=np.arange(geom.nstructures + 1), seed=K + 1)
+    out = []
+    for ahead in (0, K // 2):
+        pr = PrecisionRun(sub, x0, seed, g, u, dt, K=K, mode="mfma", ahead=ahead)
+        pr.run()
+        pr.finish()
+        out.append((pr.phase.cpu().numpy().copy(), pr.signals()))
+        pr.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_run_ahead_record_argument_rules():
     """nz_walk_record_ring refuses, before any launch: a block that wraps the
     ring, block + ahead longer than the ring, rows shorter than the ring, a
