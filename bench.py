@@ -52,6 +52,9 @@ def parse():
                     help="step-count scale of the batched/served workloads (1.0 = as stated)")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the served stream (the stated stream is 120 s)")
+    ap.add_argument("--serve-serial", type=int, default=1,
+                    help="also run the served stream's requests one at a time (0 = skip: "
+                         "about four times the served makespan)")
     ap.add_argument("--precision-spins", type=int, default=4_000_000)
     ap.add_argument("--precision-steps", type=int, default=10000,
                     help="steps of the precision-form measurement (1e4 = the full workload)")
@@ -502,22 +505,27 @@ def serving_form(args):
     got, makespan = serve_trace(srv, reqs, arr, progress=_progress("served"))
     built, reused = srv.subs.built, srv.subs.reused
     srv.close()
-    alone, t_s = run_serial(reqs, cache=SubstrateCache(dev, 64 << 30, tune=True),
-                            progress=_progress("serial"))
-    diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
     ss = total_spin_steps(reqs)
-    lat_serial = fifo_latencies(arr, [alone[r.rid].latency for r in reqs])
+    if args.serve_serial:
+        alone, t_s = run_serial(reqs, cache=SubstrateCache(dev, 64 << 30, tune=True),
+                                progress=_progress("serial"))
+        diff = max(float(np.abs(got[r].signals - alone[r].signals).max()) for r in got)
+        lat_serial = fifo_latencies(arr, [alone[r.rid].latency for r in reqs])
+        serial = {"serial_sims_per_s": len(reqs) / t_s,
+                  "serial_fifo_latency": {"p50_s": float(np.percentile(lat_serial, 50)),
+                                          "p99_s": float(np.percentile(lat_serial, 99))},
+                  "max_signal_diff_vs_alone": diff}
+    else:
+        serial = {"serial_sims_per_s": None, "serial_fifo_latency": None,
+                  "max_signal_diff_vs_alone": None}
     return {
         "requests": len(reqs), "rate_per_s": 50.0, "arrival_window_s": args.serve_seconds,
         "step_scale": args.forms_steps, "offered_spin_steps_per_s": ss / args.serve_seconds,
         "makespan_s": makespan, "served_sims_per_s": len(reqs) / makespan,
-        "served_spin_steps_per_s": ss / makespan, "serial_sims_per_s": len(reqs) / t_s,
+        "served_spin_steps_per_s": ss / makespan, **serial,
         "latency": latency_stats(got),
-        "serial_fifo_latency": {"p50_s": float(np.percentile(lat_serial, 50)),
-                                "p99_s": float(np.percentile(lat_serial, 99))},
         "substrates_built": built, "substrates_reused": reused,
         "families_tuned": len(families), "tuning_s_before_trace": t_tune,
-        "max_signal_diff_vs_alone": diff,
     }
 
 

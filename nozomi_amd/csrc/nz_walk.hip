@@ -173,6 +173,10 @@ enum WalkKind { kPlain = 0, kMoments = 1, kRecord = 2 };
 // at least P.target.  Every lane of the block calls it (inactive lanes too:
 // the per-step moment partials synchronise the block).  kMoments: per-step
 // FP64 moment sums (P.stepmom); kRecord: per-step displacement record (P.rec).
+// LDS words read volatile: at each use, not kept in registers.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const volatile f64x2 lds_f64x2;
+
 template <int kKind>
 __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, int64_t ld,
                                           int64_t i, bool active, const StepParams P,
@@ -197,23 +201,19 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
   const bool do_t2 = kTrack && (P.flags & NZ_WALK_T2) != 0;
   const uint32_t spin = active ? (uint32_t)st.spin_id[ii] : 0u;
   uint32_t t = active ? st.tstep[ii] : P.target;
-  double x0[3] = {0.0, 0.0, 0.0};
   if (kKind == kMoments) {
     for (int k = threadIdx.x; k < (int)(P.target - P.mom_base) * 6; k += blockDim.x) smom[k] = 0.0;
     __syncthreads();
   }
-  if (kKind == kMoments) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) x0[a] = active ? st.x0[a * ld + ii] : 0.0;
-  }
-  // kRecord: the start position and the record row's offset wait in LDS
-  // (xs: 4 doubles per thread, the offset's bits last) and are read at each
-  // step end; held in registers they were spilled to scratch, and each
+  // The tracking kinds keep the start position in LDS (xs: 4 doubles per
+  // thread; kRecord adds the record row's offset as the fourth) and read it
+  // at each step end; held in registers it was spilled to scratch, and each
   // reload waited behind the wave's outstanding vector loads
-  if (kKind == kRecord) {
+  if (kKind != kPlain) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) xs[4 * threadIdx.x + a] = active ? st.x0[a * ld + ii] : 0.0;
-    xs[4 * threadIdx.x + 3] = __longlong_as_double((long long)(ii * P.rec_ld));
+    xs[4 * threadIdx.x + 3] =
+        __longlong_as_double(kKind == kRecord ? (long long)(ii * P.rec_ld) : 0ll);
   }
 
   bool started = false;
@@ -249,9 +249,6 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
           // wavefront are at different steps, so rows of steps would scatter)
           uint32_t e = t - P.mom_base + P.rec_pos0;  // < 2 rec_ring
           if (e >= P.rec_ring) e -= P.rec_ring;
-          // volatile: read here, each step, not kept in registers
-          typedef double f64x2 __attribute__((ext_vector_type(2)));
-          typedef __attribute__((address_space(3))) const volatile f64x2 lds_f64x2;
           lds_f64x2* xv = (lds_f64x2*)(xs + 4 * threadIdx.x);
           const f64x2 u0 = xv[0], u1 = xv[1];
           const double xl[3] = {u0.x, u0.y, u1.x};
@@ -262,9 +259,12 @@ __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, in
         }
         if (kKind == kMoments) {  // moment sums (sim3d_kernel.cu:191-237), FP64 throughout
           const int k = (int)(t - P.mom_base) * 6;
+          const f64x2 u0 = ((lds_f64x2*)(xs + 4 * threadIdx.x))[0];
+          const double x0z = ((lds_f64x2*)(xs + 4 * threadIdx.x))[1].x;
+          const double xl[3] = {u0.x, u0.y, x0z};
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            const double d = lane_disp(g, s, x0, a);
+            const double d = lane_disp(g, s, xl, a);
             const double d2 = d * d;
             atomicAdd(&smom[k + a], d2);
             atomicAdd(&smom[k + 3 + a], d2 * d2);
@@ -321,7 +321,8 @@ __global__ void __launch_bounds__(256) NZ_TRACK_ATTR moments_kernel(const WalkAr
   const StepParams P{A.seed,  (uint32_t)A.base, (uint32_t)A.target, (uint32_t)A.limit,
                      (uint32_t)A.mom_base, A.dt, A.gph, A.garea, A.stepmom, A.flags, A.rec,
                      A.rec_ld};
-  walk_slot<kMoments>(*gpr, A.st, A.n, i, i < A.n, P, smom);
+  extern __shared__ double mom_xs[];  // 4 per thread: start position
+  walk_slot<kMoments>(*gpr, A.st, A.n, i, i < A.n, P, smom, mom_xs);
 }
 
 __global__ void __launch_bounds__(256) NZ_TRACK_ATTR record_kernel(const WalkArgs A,
@@ -1228,7 +1229,8 @@ int nz_walk(nz_walker_t ww, uint64_t seed, int32_t nsteps, double dt, const floa
       NZ_HIP(hipEventRecord(e0, s));
     }
     if (stepmom)
-      hipLaunchKernelGGL(moments_kernel, dim3(blocks_for(w->n, w->track_block)), dim3(w->track_block), 0, s, a,
+      hipLaunchKernelGGL(moments_kernel, dim3(blocks_for(w->n, w->track_block)), dim3(w->track_block),
+                         (size_t)w->track_block * 4 * sizeof(double), s, a,
                          (const Grid*)w->sub->g_dev);
     else
       hipLaunchKernelGGL(walk_kernel, dim3(blocks_for(w->n, w->walk_block)), dim3(w->walk_block), 0,
