@@ -27,6 +27,7 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP32_VECTOR_PEAK_TFLOPS = 157.3
+VALU_ISSUE_PER_S = 256 * 2.4e9  # wave64 VALU instructions: one per CU per cycle at 2.4 GHz
 
 
 def parse():
@@ -183,6 +184,11 @@ def main():
     achieved_tf = launch_units * flop_per / (launch_avg_ms * 1e-3) / 1e12
     hbm_gbs = launch_units * 24.0 / (launch_avg_ms * 1e-3) / 1e9
     traffic, traffic_note = measured_traffic(N, steps_per_launch, info)
+    # VALU issue: the counted wavefront VALU instructions of a launch (same
+    # committed summary, same sources) against the issue limit of one wave64
+    # VALU instruction per CU per cycle (no packed FP32)
+    valu = measured_traffic(N, steps_per_launch, info, key="valu_insts_per_launch")[0]
+    issue_frac = (valu / (launch_avg_ms * 1e-3 * VALU_ISSUE_PER_S) if valu else None)
 
     out = {
         "metric": "spin-steps/s at 1e6 spins",
@@ -218,6 +224,7 @@ def main():
             "traffic": traffic, "traffic_source": traffic_note, "kernel": "walk_kernel",
             "flop_per_spin_step": flop_per,
             "spin_steps_per_launch": launch_units,
+            "valu_insts_per_launch": valu, "issue_frac": issue_frac,
         },
         "roofline_hbm": {
             "bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -542,12 +549,12 @@ def walk_source_hash():
     return h.hexdigest()[:16]
 
 
-def measured_traffic(n, steps_per_launch, info):
+def measured_traffic(n, steps_per_launch, info, key="bytes_per_launch"):
     """HBM bytes per walk-kernel launch from the committed counter summary
     (profiles/walk_kernel_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch,
-    scaled by the copy-kernel calibration), when it was measured on this
-    workload shape and on the walk kernel built from the current sources;
-    else None.  Returns (bytes or None, why)."""
+    scaled by the copy-kernel calibration), or another of its figures (key),
+    when it was measured on this workload shape and on the walk kernel built
+    from the current sources; else None.  Returns (value or None, why)."""
     path = os.path.join(ROOT, "profiles", "walk_kernel_traffic.json")
     try:
         with open(path) as f:
@@ -559,7 +566,7 @@ def measured_traffic(n, steps_per_launch, info):
         return None, "counter summary measured on another workload shape"
     if t.get("sources_sha16") != walk_source_hash():
         return None, "counter summary measured on other walk-kernel sources"
-    return t.get("bytes_per_launch"), t.get("source")
+    return t.get(key), t.get("source")
 
 
 def measure_sbar(geom, sub, prot, dev, seed, n=65536, steps=200):
