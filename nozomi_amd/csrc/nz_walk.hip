@@ -169,14 +169,15 @@ struct StepParams {
 // walk and the record leave the other kinds' registers to the scan.
 enum WalkKind { kPlain = 0, kMoments = 1, kRecord = 2 };
 
-// Walk the spin held in slot i (fields strided by ld) from its next step to
-// at least P.target.  Every lane of the block calls it (inactive lanes too:
-// the per-step moment partials synchronise the block).  kMoments: per-step
-// FP64 moment sums (P.stepmom); kRecord: per-step displacement record (P.rec).
 // LDS words read volatile: at each use, not kept in registers.
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) const volatile f64x2 lds_f64x2;
 
+// Walk the spin held in slot i (fields strided by ld) from its next step to
+// at least P.target.  Every lane of the block calls it (inactive lanes too:
+// the per-step moment partials synchronise the block).  kMoments: per-step
+// FP64 moment sums (P.stepmom); kRecord: per-step displacement record (P.rec).
+// xs: the tracking kinds' LDS words, 4 doubles per thread.
 template <int kKind>
 __device__ __forceinline__ void walk_slot(const Grid& g, const SlotState& st, int64_t ld,
                                           int64_t i, bool active, const StepParams P,
