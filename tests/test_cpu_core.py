@@ -98,6 +98,32 @@ def test_oracle_is_chaotic(c0):
     assert np.mean(err[res[0][1] <= 10] < 1e-7) > 0.99   # collision-poor spins do not
 
 
+def test_fp32_state_breaks_the_trajectory_bar(c0):
+    """What an FP32 walk state would cost: the FP64 oracle with its positions
+    rounded to binary32 after every step (a milder error than FP32 arithmetic
+    throughout) against the plain FP64 oracle over 1000 steps.  Most
+    collision-rich spins end more than 1e-5 um apart, the per-spin bar the
+    FP64-state kernel meets bit for bit."""
+    g, o = c0
+    N, T, dt, seed = 2048, 1000, 0.02, 1234
+    p = o.seed(N, np.arange(g.nstructures + 1), seed=5).astype(np.float32).astype(np.float64)
+    ref, ref0 = p.copy(), p.copy()
+    nsub = np.zeros(N, np.int64)
+    o.walk(ref, ref0, np.ones(N), seed=seed, step0=0, nsteps=T, dt=dt, nsub=nsub)
+    pos, pos0 = p.copy(), p.copy()
+    for t in range(T):
+        o.walk(pos, pos0, np.ones(N), seed=seed, step0=t, nsteps=1, dt=dt)
+        pos = pos.astype(np.float32).astype(np.float64)
+    err = np.abs(_min_image(pos - ref, g.Lx)).max(0)
+    many = nsub > 200
+    frac_all, frac_many = np.mean(err < 1e-5), np.mean(err[many] < 1e-5)
+    print(f"FP32 state: within 1e-5 um after {T} steps: all {frac_all:.3f}, "
+          f"spins with > 200 sub-steps {frac_many:.3f} ({many.sum()} spins)")
+    assert many.sum() > 20
+    assert frac_many < 0.5
+    assert frac_all < 0.9
+
+
 def _probes(g, rng, n_u, n_s):
     """Uniform positions and positions on and near sphere surfaces (inside
     and outside the tolerance shells), wrapped into the box."""
