@@ -107,6 +107,39 @@ def test_trajectory_parity_1000_steps(c0_geometry, oracle_lib, cell):
     assert np.abs(gs[ok] / sig[ok] - 1).max() < 1e-5
 
 
+def test_ragged_spin_counts(c0_geometry, oracle_lib):
+    """Spin counts that leave the last wavefront part-filled (1, 63, 65, 4097)
+    or fill less than one wavefront: 300 steps (a spatial re-sort at 250)
+    against the oracle, bit for bit; a zero-step walk changes nothing and a
+    negative one is refused."""
+    g = c0_geometry
+    orc = _oracle_for(g, oracle_lib)
+    dt, L = 0.02, g.Lx
+    for N in (1, 63, 65, 4097):
+        print(f"ragged N={N}", flush=True)
+        sim = DiffSim3d(g, N, counters=True)
+        sim.setup(initstates_int=77 + N)
+        pos = sim.spins_d.get().astype(np.float64)
+        pos0 = pos.copy()
+        sig = np.ones(N)
+        nsub_o = np.zeros(N, dtype=np.int64)
+        sim.run(0, dt)
+        gp, gd = sim.walker.read_f64()
+        assert np.array_equal(gp, pos) and not gd.any()
+        for c in range(2):
+            sim.run(150, dt)
+            orc.walk(pos, pos0, sig, seed=77 + N, step0=c * 150, nsteps=150, dt=dt, nsub=nsub_o)
+        gp, gd = sim.walker.read_f64()
+        nsub_g, _ = sim.walker.counters()
+        err = np.maximum(np.abs(_min_image(gp - pos, L)).max(axis=0),
+                         np.abs(gd - (pos - pos0)).max(axis=0))
+        print(f"  bit-identical {np.mean(err == 0):.4f} max {err.max():.2e}", flush=True)
+        assert np.mean(err < TRAJ_TOL) >= (TRAJ_FRAC if N > 1000 else 1.0)
+        assert np.array_equal(nsub_g, nsub_o) or N > 1000
+        with pytest.raises((ValueError, RuntimeError)):
+            sim.run(-1, dt)
+
+
 @pytest.mark.parametrize("dt,nsteps", [(0.3, 40), (1.0, 20)])
 def test_large_steps_fp64_fallbacks(c0_geometry, oracle_lib, dt, nsteps):
     """Jumps of 1.1-2 um rms (beyond the 0.8 um list margin): most sub-steps
