@@ -78,11 +78,10 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // Each wave's loads retire in issue order (vmcnt), so HBM latency (the phase
 // rows) and L2 latency (the waveform chunks) sit in different waves, and the
 // MFMA waves issue no loads after their A operand.  After a tile's chunks the
-// parts' tile sums are added into the old phases in a fixed order (part 0
-// first) and the rows stored back, 128 contiguous bytes per row: with one or
-// two parts from an LDS stage during the next tile's first chunk interval,
-// with three parts in place in LDS, one part after the other.  Barrier per
-// chunk interval (s_barrier, every wave the same count).
+// parts' tile sums wait in an LDS stage; during the next tile's first chunk
+// interval they are added into the old phases in a fixed order (part 0
+// first) and the rows stored back, 128 contiguous bytes per row.  Barrier
+// per chunk interval (s_barrier, every wave the same count).
 // Measured at 4e6 slots x 1024 waveforms: v_mfma_f32_32x32x16_bf16 sustains
 // ~2.0 PFLOP/s against 1.2 for v_mfma_f32_16x16x32_bf16, and the 32-row tile
 // halves the LDS reads per MFMA.
@@ -155,17 +154,17 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
   constexpr int kEntry = KS * 2 * kTileW * kChunk;  // uint16 per ring entry
   constexpr int kEntryLoads = KS * 2 * (kChunk / 16);  // 1 KB loads per ring entry
   constexpr int kTileWords = kTileS * kTileW;
-  // One or two parts (K = 32, 64, 128): each part writes its tile sums to an
-  // LDS stage at the end of the tile, and in the first chunk interval of the
-  // next tile every MFMA wave adds old + part 0 (+ part 1) for a share of the
-  // rows and stores them, beside that tile's MFMAs instead of behind KS
-  // barriers of its own (K = 64 8.7 -> 7.7 ms per block, K = 128 12.8 -> 12.0
-  // ms; the same additions in the same order, so the same bits).
-  constexpr bool kStaged = KS <= 2;
-  static_assert(!kStaged || NKC >= 2, "the staged epilogue needs two chunk intervals per tile");
+  // Each part writes its tile sums to an LDS stage at the end of the tile,
+  // and in the first chunk interval of the next tile every MFMA wave adds
+  // old + part 0 + part 1 ... for a share of the rows and stores them, beside
+  // that tile's MFMAs (a per-part epilogue, each part adding its tile into
+  // the old phases in LDS behind a barrier of its own, ran K = 64 8.7, K = 96
+  // 11.9, K = 128 12.8 ms per block against 7.7, 10.4, 12.0 ms; the same
+  // additions in the same order, so the same bits).
+  static_assert(NKC >= 2, "the staged epilogue needs two chunk intervals per tile");
   __shared__ __attribute__((aligned(16))) uint16_t sB[kRing * kEntry];
   __shared__ __attribute__((aligned(16))) float sP[2 * kTileWords];
-  __shared__ __attribute__((aligned(16))) float sS[kStaged ? KS * kTileWords : 4];
+  __shared__ __attribute__((aligned(16))) float sS[KS * kTileWords];
   const int64_t s0 = (int64_t)blockIdx.x * kTileS;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ntiles = (W + kTileW - 1) / kTileW;
@@ -208,13 +207,13 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
     int bo[kChunk / 16];  // the lane's B piece of 16-k step s: k = 16 s + kh of row col
 #pragma unroll
     for (int s = 0; s < kChunk / 16; ++s) bo[s] = col * kChunk + 8 * ((2 * s + (kh >> 3)) ^ bswz<CH>(col));
-    // kStaged: old phases + the staged part sums of tile u, rows
+    // old phases + the staged part sums of tile u, rows
     // 32 sg + 8 j + l / 8 for this wave's pieces j, stored back
     auto combine = [&](int u) {
       const float* tp = sP + (u & 1) * kTileWords;
 #pragma unroll
-      for (int jj = 0; jj < 4 / KS; ++jj) {
-        const int j = kq * (4 / KS) + jj;
+      for (int j = 0; j < 4; ++j) {
+        if (j < (4 * kq) / KS || j >= (4 * (kq + 1)) / KS) continue;  // the pieces of part kq
         const int off = (sg * 32 + 8 * j) * kTileW + 4 * lane;
         floatx4 v;
         __builtin_memcpy(&v, tp + off, 16);
@@ -253,38 +252,16 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a0 + 1], l1, acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a0 + 1], h1, acc, 0, 0, 0);
         }
-        if constexpr (kStaged) {
-          if (kc == 0 && t >= 1) combine(t - 1);
-          if (kc == NKC - 1) {
-#pragma unroll
-            for (int v = 0; v < 16; ++v)
-              sS[kq * kTileWords + pword(sg * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), col)] = acc[v];
-          }
-        }
-        wg_barrier();
-      }
-      if constexpr (kStaged) continue;
-      float* tp = sP + (t & 1) * kTileWords;
-#pragma unroll
-      for (int q = 0; q < KS; ++q) {
-        if (kq == q) {
+        if (kc == 0 && t >= 1) combine(t - 1);
+        if (kc == NKC - 1) {
 #pragma unroll
           for (int v = 0; v < 16; ++v)
-            tp[pword(sg * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), col)] += acc[v];
-          if (q == KS - 1) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              floatx4 v;
-              __builtin_memcpy(&v, tp + (sg * 32 + 8 * j) * kTileW + 4 * lane, 16);
-              const int c = t * kTileW + pcol;
-              if (psp[j] >= 0 && c < W) __builtin_nontemporal_store(v, (floatx4*)(phase + (int64_t)psp[j] * W + c));
-            }
-          }
+            sS[kq * kTileWords + pword(sg * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), col)] = acc[v];
         }
         wg_barrier();
       }
     }
-    if constexpr (kStaged) combine(ntiles - 1);
+    combine(ntiles - 1);
   } else if (wave == 4 * KS) {  // ---- waveform wave: ring entries
     // entry of interval gi: chunk gi % NKC of tile gi / NKC, all KS parts
     auto issue = [&](int gi) {
@@ -322,8 +299,6 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
         }
         wg_barrier();
       }
-      if constexpr (!kStaged)
-        for (int q = 0; q < KS; ++q) wg_barrier();
     }
   } else {  // ---- phase wave: old phases of tile t into sP[t % 2]
     // the phases are read and written once per block: non-temporal loads
@@ -350,14 +325,12 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
     wg_barrier();
     for (int t = 0; t < ntiles; ++t) {
       for (int kc = 0; kc < NKC; ++kc) {
-        // sP[(t + 1) % 2] was last read in tile t - 1's epilogue, which ends
-        // with tile t - 1 (kStaged: with the first chunk interval of tile t)
-        if (kc == (kStaged ? 1 : 0) && t >= 1 && t + 1 < ntiles) issue(t + 1);
+        // sP[(t + 1) % 2] was last read in tile t - 1's epilogue, in the
+        // first chunk interval of tile t
+        if (kc == 1 && t >= 1 && t + 1 < ntiles) issue(t + 1);
         if (kc == NKC - 1) wait_vmcnt(t + 1 < ntiles ? 16 : 0);  // tile t landed
         wg_barrier();
       }
-      if constexpr (!kStaged)
-        for (int q = 0; q < KS; ++q) wg_barrier();
     }
   }
 }
