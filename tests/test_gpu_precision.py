@@ -15,10 +15,14 @@ pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
 
 
+@pytest.mark.parametrize("W", [200, 24])
 @pytest.mark.parametrize("K", [32, 64, 96, 128])
-def test_gemm_modes_against_fp64(K):
-    rng = np.random.default_rng(K)
-    N, W = 1000, 200
+def test_gemm_modes_against_fp64(K, W):
+    """200 waveforms: seven 32-waveform tiles, the last part-filled; 24: one
+    tile (its epilogue only after the tile loop).  1000 slots: the last
+    128-slot workgroup part-filled, every seventh slot padding."""
+    rng = np.random.default_rng(K + W)
+    N = 1000
     rec = rng.normal(scale=5.0, size=(N, 3 * K)).astype(np.float32)  # slot-major
     bt = rng.normal(scale=1e-3, size=(W, 3 * K)).astype(np.float32)
     slot_spin = rng.permutation(N).astype(np.int32)
