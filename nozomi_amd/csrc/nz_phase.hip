@@ -78,9 +78,9 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // Each wave's loads retire in issue order (vmcnt), so HBM latency (the phase
 // rows) and L2 latency (the waveform chunks) sit in different waves, and the
 // MFMA waves issue no loads after their A operand.  After a tile's chunks the
-// parts' tile sums wait in an LDS stage; during the next tile's first chunk
-// interval they are added into the old phases in a fixed order (part 0
-// first) and the rows stored back, 128 contiguous bytes per row.  Barrier
+// parts' tile sums wait in an LDS stage; during the next tile's first two
+// chunk intervals they are added into the old phases in a fixed order (part
+// 0 first) and the rows stored back, 128 contiguous bytes per row.  Barrier
 // per chunk interval (s_barrier, every wave the same count).
 // Measured at 4e6 slots x 1024 waveforms: v_mfma_f32_32x32x16_bf16 sustains
 // ~2.0 PFLOP/s against 1.2 for v_mfma_f32_16x16x32_bf16, and the 32-row tile
@@ -155,13 +155,14 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
   constexpr int kEntryLoads = KS * 2 * (kChunk / 16);  // 1 KB loads per ring entry
   constexpr int kTileWords = kTileS * kTileW;
   // Each part writes its tile sums to an LDS stage at the end of the tile,
-  // and in the first chunk interval of the next tile every MFMA wave adds
-  // old + part 0 + part 1 ... for a share of the rows and stores them, beside
-  // that tile's MFMAs (a per-part epilogue, each part adding its tile into
-  // the old phases in LDS behind a barrier of its own, ran K = 64 8.7, K = 96
-  // 11.9, K = 128 12.8 ms per block against 7.7, 10.4, 12.0 ms; the same
-  // additions in the same order, so the same bits).
-  static_assert(NKC >= 2, "the staged epilogue needs two chunk intervals per tile");
+  // and in the first two chunk intervals of the next tile every MFMA wave
+  // adds old + part 0 + part 1 ... for its share of the rows, half of it per
+  // interval, and stores them, beside that tile's MFMAs (a per-part
+  // epilogue, each part adding its tile into the old phases in LDS behind a
+  // barrier of its own, ran K = 64 8.7, K = 96 11.9, K = 128 12.8 ms per
+  // block against 7.4, 10.4, 11.1 ms; the same additions in the same order,
+  // so the same bits).
+  static_assert(NKC >= 3, "the staged epilogue needs three chunk intervals per tile");
   __shared__ __attribute__((aligned(16))) uint16_t sB[kRing * kEntry];
   __shared__ __attribute__((aligned(16))) float sP[2 * kTileWords];
   __shared__ __attribute__((aligned(16))) float sS[KS * kTileWords];
@@ -208,12 +209,14 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
 #pragma unroll
     for (int s = 0; s < kChunk / 16; ++s) bo[s] = col * kChunk + 8 * ((2 * s + (kh >> 3)) ^ bswz<CH>(col));
     // old phases + the staged part sums of tile u, rows
-    // 32 sg + 8 j + l / 8 for this wave's pieces j, stored back
-    auto combine = [&](int u) {
+    // 32 sg + 8 j + l / 8 for this wave's pieces j (half 0: the first half of
+    // them, 1: the rest, 2: all), stored back
+    auto combine = [&](int u, int half) {
       const float* tp = sP + (u & 1) * kTileWords;
+      const int lo = (4 * kq) / KS, hi = (4 * (kq + 1)) / KS, mid = lo + (hi - lo + 1) / 2;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        if (j < (4 * kq) / KS || j >= (4 * (kq + 1)) / KS) continue;  // the pieces of part kq
+        if (half == 0 ? (j < lo || j >= mid) : half == 1 ? (j < mid || j >= hi) : (j < lo || j >= hi)) continue;
         const int off = (sg * 32 + 8 * j) * kTileW + 4 * lane;
         floatx4 v;
         __builtin_memcpy(&v, tp + off, 16);
@@ -252,7 +255,7 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a0 + 1], l1, acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a0 + 1], h1, acc, 0, 0, 0);
         }
-        if (kc == 0 && t >= 1) combine(t - 1);
+        if (kc < 2 && t >= 1) combine(t - 1, kc);
         if (kc == NKC - 1) {
 #pragma unroll
           for (int v = 0; v < 16; ++v)
@@ -261,7 +264,7 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
         wg_barrier();
       }
     }
-    combine(ntiles - 1);
+    combine(ntiles - 1, 2);
   } else if (wave == 4 * KS) {  // ---- waveform wave: ring entries
     // entry of interval gi: chunk gi % NKC of tile gi / NKC, all KS parts
     auto issue = [&](int gi) {
@@ -326,8 +329,8 @@ phase_gemm_mfma_kernel(const float* rec, int64_t ld, int64_t nslots, const int32
     for (int t = 0; t < ntiles; ++t) {
       for (int kc = 0; kc < NKC; ++kc) {
         // sP[(t + 1) % 2] was last read in tile t - 1's epilogue, in the
-        // first chunk interval of tile t
-        if (kc == 1 && t >= 1 && t + 1 < ntiles) issue(t + 1);
+        // first two chunk intervals of tile t
+        if (kc == 2 && t >= 1 && t + 1 < ntiles) issue(t + 1);
         if (kc == NKC - 1) wait_vmcnt(t + 1 < ntiles ? 16 : 0);  // tile t landed
         wg_barrier();
       }
